@@ -1,0 +1,197 @@
+#!/usr/bin/env python
+"""Benchmark: streamline topology over a 5,000-charge frame on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+One "step" is one full pass of the streamline hot path (order, propagate,
+FP64 epilogue) over 100,000 starts with inputs already resident in HBM:
+5,000 charges (35 A ball, 3 A exclusion around the +-1.5 A box), h = 0.001 A,
+Euler, n_iter = randint(1, max_steps) (seed 42) capped at 2,000.
+
+Prints ONE JSON line.  `value` is charge-point pair-evals/s, counted as
+(Euler steps + 4 curvature steps) x charges per streamline, as the reference
+does its work (thread_operation, CPET/utils/math_module.c:523-591).
+`roofline` is for the dominant kernel (topo_propagate) from HIP events on
+the stream it runs on: 18 FLOP per charge-point pair x charges x Euler steps
+per launch / the kernel's mean duration, against the FP32 vector peak.
+`cpu_baseline` times the reference's own thread_operation (built from its
+source under oracle/_ref) on a bounded seeded sample of the same streamlines
+on the host cores, or the oracle's FP32 port when that build is absent.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "charge-point pair-evals/s + streamlines/s; frames/s and p50/p99 latency"
+PEAK_FP32_TFLOPS = 157.3          # MI355X FP32 vector, AMD-published
+PEAK_FP32_UNPACKED_TFLOPS = 78.6  # without packed FP32 (disabled in this build)
+FLOP_PER_PAIR = 18
+
+
+def cpu_baseline(case, gpu_steps, seconds: float, cores: int):
+    """Reference thread_operation on a seeded sample of the same streamlines."""
+    from oracle import oracle as O
+
+    rng = np.random.default_rng(123)
+    order = rng.permutation(case.L)
+    if O.Ref.available():
+        ref = O.Ref()
+        kind = "reference"
+        run_one = lambda i: ref.thread_operation(case.starts[i], case.n_iter[i], case.h,
+                                                 case.dims, case.x, case.q)
+    else:
+        kind = "port"
+        run_one = lambda i: O.topo_f32(case.starts[i:i + 1], case.n_iter[i:i + 1], case.h,
+                                       case.dims, case.x, case.q)
+    from concurrent.futures import ThreadPoolExecutor
+
+    done = []
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=cores) as ex:
+        pos = 0
+        while time.perf_counter() - t0 < seconds and pos < case.L:
+            batch = order[pos:pos + 4 * cores]
+            list(ex.map(run_one, batch))
+            done.extend(batch.tolist())
+            pos += len(batch)
+    dt = time.perf_counter() - t0
+    done = np.array(done)
+    pairs = float((gpu_steps[done].astype(np.int64) + 4).sum()) * case.N
+    return {
+        "value": pairs / dt,
+        "unit": "pair-evals/s",
+        "cores": cores,
+        "kind": kind,
+        "sample": f"{len(done)} of {case.L} streamlines (seeded), {dt:.1f} s wall; "
+                  "pair-evals counted with the GPU's per-streamline step counts",
+        "streamlines_per_s": len(done) / dt,
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-volume", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU sharding is a design note, not a measured path", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from pycpet_amd import _native as nat, ops, workloads as W
+
+    dev = torch.device("cuda:0")
+    case = W.topo_5k()
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    starts, n_iter, dims, x, q = T(case.starts), T(case.n_iter), T(case.dims), T(case.x), T(case.q)
+    ctx = nat.Context(0)
+    ctx.set_profiling(True)
+
+    for _ in range(args.warmup):
+        r = ops.topo(starts, n_iter, case.h, dims, x, q, ctx=ctx)
+    torch.cuda.synchronize()
+
+    prop_ms = []
+    epi_ms = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = ops.topo(starts, n_iter, case.h, dims, x, q, ctx=ctx)
+        ph = ctx.phase_times(3)  # waits for this step's last event
+        prop_ms.append(ph[1])
+        epi_ms.append(ph[2])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+
+    steps_np = r.steps.cpu().numpy()
+    euler_steps = int(steps_np.astype(np.int64).sum())
+    pairs_per_step = float(euler_steps + 4 * case.L) * case.N
+    ms_per_step = 1e3 * wall / args.steps
+    value = pairs_per_step * args.steps / wall
+    prop = float(np.mean(prop_ms))
+    achieved = FLOP_PER_PAIR * case.N * euler_steps / (prop * 1e-3) / 1e12
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "pair-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (FP64 positions and curvature steps)",
+        "data": "synthetic",
+        "config": {"workload": "topo_5k_charges_100k_streamlines", "charges": case.N,
+                   "streamlines": case.L, "h": case.h, "box_half_width": 1.5,
+                   "n_iter": "randint(1, 5196) seed 42, capped at 2000",
+                   "integrator": "euler (reference)", "euler_steps": euler_steps},
+        "streamlines_per_s": case.L * args.steps / wall,
+        "roofline": {
+            "bound": "valu",
+            "kernel": "topo_propagate",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "frac_of_unpacked_peak": achieved / PEAK_FP32_UNPACKED_TFLOPS,
+            "traffic": None,
+            "kernel_ms": prop,
+            "epilogue_ms": float(np.mean(epi_ms)),
+            "flop_per_pair": FLOP_PER_PAIR,
+        },
+    }
+
+    if not args.no_volume:
+        vc = W.volume_30k()
+        ax = [T(a) for a in vc.axes]
+        vx, vq = T(vc.x), T(vc.q)
+        for _ in range(2):
+            ops.field_grid(*ax, vx, vq, ctx=ctx)
+        torch.cuda.synchronize()
+        kms = []
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            ops.field_grid(*ax, vx, vq, ctx=ctx)
+            kms.append(ctx.phase_times(2)[0])
+        torch.cuda.synchronize()
+        vwall = (time.perf_counter() - t1) / args.steps
+        vpairs = float(vc.M) * vc.N
+        out["volume_30k"] = {
+            "pair_evals_per_s": vpairs / vwall,
+            "ms_per_frame": 1e3 * vwall,
+            "grid_points": vc.M, "charges": vc.N,
+            "kernel_ms": float(np.mean(kms)),
+            "achieved_tflops": FLOP_PER_PAIR * vpairs / (np.mean(kms) * 1e-3) / 1e12,
+        }
+
+    if not args.no_cpu_baseline and int(os.environ.get("RANK", "0")) == 0:
+        cores = min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(case, steps_np, args.cpu_seconds, cores)
+    else:
+        out["cpu_baseline"] = None
+
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

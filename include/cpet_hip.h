@@ -1,0 +1,170 @@
+/*
+ * libcpet_hip.so — C-ABI of the MI355X (gfx950) implementation of PyCPET's
+ * two data-parallel hot paths:
+ *   1. the direct Coulomb field / electrostatic potential over box grids
+ *      (options-file methods "volume", "volume_ESP"), and
+ *   2. electric-field streamline propagation with distance / curvature
+ *      features and a 2-D histogram ("topo", "topo_GPU").
+ *
+ * Every entry point returns an int status: 0 on success, a negative
+ * CPET_ERR_* code otherwise; cpet_last_error() then describes the failure
+ * (thread-local).  Nothing aborts the process.
+ *
+ * Part 1 keeps the exact names, argument order and argument meaning of the
+ * reference's native module CPET/utils/math_module.c, so the reference's
+ * ctypes binding (CPET/utils/c_ops.py: CDLL + ndpointer argtypes, restype
+ * None) binds this library unchanged.  Those entry points take host arrays.
+ *
+ * Part 2 is the batched, device-resident API: a context (device + stream +
+ * scratch) and whole-grid / whole-streamline-set calls whose array arguments
+ * are host pointers, or device pointers when CPET_F_DEVICE is set (for
+ * example torch.Tensor.data_ptr() of tensors on that device).
+ */
+#ifndef CPET_HIP_H
+#define CPET_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define CPET_OK 0
+#define CPET_ERR_ARG (-1)    /* invalid argument (sizes, NULLs, non-finite) */
+#define CPET_ERR_HIP (-2)    /* a HIP runtime call failed */
+#define CPET_ERR_NODEV (-3)  /* no usable GPU */
+#define CPET_ERR_NOMEM (-4)  /* device allocation failed */
+#define CPET_ERR_BUDGET (-5) /* request exceeds the admission budget */
+
+/* ----------------------------------------------------------------- flags */
+#define CPET_F_DEVICE 0x1u   /* array arguments are device pointers */
+#define CPET_F_OUT_F64 0x2u  /* field/phi outputs are double, not float */
+#define CPET_F_NO_SYNC 0x4u  /* device-pointer calls: do not wait for the stream */
+/* precision of the FP32 field kernels, bits 8..9 */
+#define CPET_F_PREC_TILED (0u << 8) /* FP32 per 64-charge tile, FP64 across (default) */
+#define CPET_F_PREC_NAIVE (1u << 8) /* one FP32 running sum */
+#define CPET_F_PREC_KAHAN (2u << 8) /* Kahan FP32 within a tile, FP64 across */
+#define CPET_F_PREC_F64 (3u << 8)   /* FP64 throughout */
+#define CPET_F_PREC_MASK (3u << 8)
+
+/* ============ Part 1: reference-compatible entry points (host arrays) ==== */
+
+/* E[m] = k * sum_j Q[j] (x_0[m] - x[j]) * max(|x_0[m]-x[j]|^2, 1e-6)^(-3/2)
+ * for m < total_points.  x_0 (total_points,3), x (n_charges,3), Q
+ * (n_charges), E (total_points,3), all float32 C-order.
+ * Replaces compute_looped_field, CPET/utils/math_module.c:405-451
+ * (binding CPET/utils/c_ops.py:151-159, wrapper :250-263). */
+int compute_looped_field(int total_points, int n_charges, const float *x_0,
+                         const float *x, const float *Q, float *E);
+
+/* ESP[0] += k * sum_j Q[j] / |x_init - x[j]| (no softening; r = 0 gives inf).
+ * Replaces calc_esp_base, CPET/utils/math_module.c:453-486
+ * (binding CPET/utils/c_ops.py:131-138, wrapper :326-341). */
+int calc_esp_base(float *ESP, const float *x_init, int n_charges,
+                  const float *x, const float *Q);
+
+/* E = k * sum_j Q[j] (x_init - x[j]) / |x_init - x[j]|^3 (no softening).
+ * Replaces calc_field, CPET/utils/math_module.c:255-293
+ * (binding CPET/utils/c_ops.py:113-120, wrapper :360-375). */
+int calc_field(float *E, const float *x_init, int n_charges, const float *x,
+               const float *Q);
+
+/* As calc_field but accumulates into E (E += field), as the reference does.
+ * Replaces calc_field_base, CPET/utils/math_module.c:296-333
+ * (binding CPET/utils/c_ops.py:122-129, wrapper :343-358). */
+int calc_field_base(float *E, const float *x_init, int n_charges,
+                    const float *x, const float *Q);
+
+/* One streamline from x_0: up to n_iter Euler steps x += h E/|E|, stopping
+ * after the first point strictly outside +-dimensions; ret = [distance,
+ * mean curvature] (float32).  Replaces thread_operation,
+ * CPET/utils/math_module.c:523-591 (binding CPET/utils/c_ops.py:85-95,
+ * wrapper :281-302). */
+int thread_operation(int n_charges, int n_iter, float step_size,
+                     const float *x_0, const float *dimensions,
+                     const float *x, const float *Q, float *ret);
+
+/* ============ Part 2: batched / device-resident API ====================== */
+
+typedef struct cpet_ctx cpet_ctx;
+
+/* Description of the last failure on this thread ("" if none). */
+const char *cpet_last_error(void);
+/* Number of visible GPUs (0 when none). */
+int cpet_device_count(int *n);
+
+/* A context owns one device, one HIP stream and grow-only scratch buffers.
+ * Calls on one context are serialised; contexts run concurrently.
+ * No counterpart in the reference (which has no device state). */
+int cpet_ctx_create(int device, cpet_ctx **out);
+int cpet_ctx_destroy(cpet_ctx *ctx);
+/* Use the caller's stream (e.g. torch.cuda.current_stream().cuda_stream)
+ * instead of the context's own; NULL restores the own stream. */
+int cpet_ctx_set_stream(cpet_ctx *ctx, void *hip_stream);
+int cpet_ctx_get_stream(cpet_ctx *ctx, void **hip_stream);
+int cpet_ctx_sync(cpet_ctx *ctx);
+
+/* Field and/or potential at M arbitrary points pts (M,3).
+ *   E_out  (M,3) or NULL: softened field as compute_looped_field
+ *                  (math_module.c:405-451) when soften != 0, else the
+ *                  unsoftened field of calc_field (math_module.c:255-293);
+ *   phi_out (M)  or NULL: k sum q/r as calc_esp_base (math_module.c:453-486).
+ * Outputs are float, or double with CPET_F_OUT_F64.
+ * Batched replacement of compute_field_on_grid / compute_ESP_on_grid
+ * (CPET/utils/calculator.py:430-475), which call the two C kernels above. */
+int cpet_field_points(cpet_ctx *ctx, int M, int N, const float *pts,
+                      const float *x, const float *q, void *E_out,
+                      void *phi_out, int soften, uint32_t flags);
+
+/* The same on the tensor-product grid ax[nx] x ay[ny] x az[nz] with
+ * row-major ('ij', z fastest) point order — the mesh built by
+ * initialize_box_points_uniform (CPET/utils/calculator.py:171-248) and
+ * flattened by compute_field_on_grid (CPET/utils/calculator.py:442).
+ * Same per-pair arithmetic as cpet_field_points on the flattened mesh; the
+ * two agree to FP32 rounding (bitwise when both use the same charge split). */
+int cpet_field_grid(cpet_ctx *ctx, int nx, int ny, int nz, const float *ax,
+                    const float *ay, const float *az, int N, const float *x,
+                    const float *q, void *E_out, void *phi_out, int soften,
+                    uint32_t flags);
+
+/* L streamlines (thread_operation semantics, math_module.c:523-591; the
+ * topo_GPU path CPET/source/calculator.py:793-978 computes the same).
+ *   starts (L,3) float, n_iter (L) int32, h, dims (3) float
+ *   feats  (L,2) double out: [distance, mean curvature] per start, input order
+ *   steps  (L) int32 out or NULL: Euler steps taken
+ *   exited (L) int32 out or NULL: 1 if stopped by the box, 0 by n_iter
+ *   hist   (nb_d*nb_c) uint32 out or NULL: 2-D histogram of the float32-
+ *          rounded features with numpy.histogramdd edge rules over the
+ *          given edges (nb_d+1 and nb_c+1 doubles, increasing), as
+ *          np.histogram2d does in CPET/utils/calculator.py:702-707.
+ * Positions are integrated in FP64 with FP32 tiled field sums; the four
+ * curvature-defining evaluations run in FP64. */
+int cpet_topo(cpet_ctx *ctx, int N, int L, const float *starts,
+              const int32_t *n_iter, float h, const float *dims,
+              const float *x, const float *q, double *feats, int32_t *steps,
+              int32_t *exited, int nb_d, const double *edges_d, int nb_c,
+              const double *edges_c, uint32_t *hist, uint32_t flags);
+
+/* Opt-in per-phase timing with HIP events recorded on the context's stream
+ * (the stream the kernels run on).  After a call returns (or after
+ * cpet_ctx_sync for CPET_F_NO_SYNC calls), cpet_ctx_phase_times gives the
+ * durations in ms of the last call's phases:
+ *   cpet_topo:          [order, propagate, epilogue(+histogram)]
+ *   cpet_field_*:       [field kernel, split reduction]
+ * No counterpart in the reference (its timing is time.time() around calls,
+ * CPET/source/calculator.py:686,706-711). */
+int cpet_ctx_set_profiling(cpet_ctx *ctx, int on);
+int cpet_ctx_phase_times(cpet_ctx *ctx, float *ms, int n);
+
+/* Tuning knobs (autotuner): 0 restores the built-in heuristic.
+ *   cpet_set_tuning(kind, a, b): kind 0 = grid field (a = points per lane
+ *   along z, b = charge splits), kind 1 = points field (a = points per lane,
+ *   b = splits), kind 2 = topo (a = waves per workgroup, b = workgroups). */
+int cpet_set_tuning(cpet_ctx *ctx, int kind, int a, int b);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CPET_HIP_H */

@@ -1,0 +1,294 @@
+"""ORACLE — test infrastructure, not product code.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+import this module, and only as the checker / the CPU baseline.  The product
+(pycpet_amd/) never imports it.
+
+Contents
+  field_f64 / topo_f64      ctypes front-ends of liboracle.so (cpet_oracle.c),
+                            the FP64 restatement the GPU is measured against.
+  field_f32 / topo_f32      FP32 port of the reference arithmetic (restated
+                            CPU baseline).
+  Ref                       ctypes front-end of oracle/_ref/libref_math.so,
+                            the reference's own math_module.c compiled from its
+                            source (present only where the reference exists at
+                            build time; the built .so may travel with the tree).
+  grid_axes, histogram2d, filter_* , box_basis
+                            numpy restatements of the host-side pieces.
+
+Parity pins (where each function is checked) are listed per function; a
+function with no pin would say "parity unpinned" here.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ORACLE_SO = HERE / "liboracle.so"
+REF_SO = HERE / "_ref" / "libref_math.so"
+
+_P = ctypes.c_void_p
+_i = ctypes.c_int
+_f = ctypes.c_float
+_lib = None
+
+
+def lib():
+    """liboracle.so; built by `make -C oracle` (or __graft_entry__.build())."""
+    global _lib
+    if _lib is None:
+        if not ORACLE_SO.exists():
+            raise RuntimeError(f"{ORACLE_SO} missing: run `make -C oracle oracle`")
+        L = ctypes.CDLL(str(ORACLE_SO))
+        L.oracle_field_f64.argtypes = [_i, _i, _P, _P, _P, _i, _P, _P]
+        L.oracle_field_f32.argtypes = [_i, _i, _P, _P, _P, _i, _P, _P]
+        L.oracle_topo_f64.argtypes = [_i, _i, _P, _P, _f, _P, _P, _P, _P, _P, _P, _P]
+        L.oracle_topo_f32.argtypes = [_i, _i, _P, _P, _f, _P, _P, _P, _P, _P]
+        for fn in (L.oracle_field_f64, L.oracle_field_f32, L.oracle_topo_f64, L.oracle_topo_f32):
+            fn.restype = None
+        _lib = L
+    return _lib
+
+
+def _f32(a, last=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a.reshape(-1, last) if last else a.ravel()
+
+
+def _p(a):
+    return a.ctypes.data if a is not None else None
+
+
+def field_f64(pts, x, q, *, want_E=True, want_phi=False, soften=True):
+    """FP64 E (softened as compute_looped_field, math_module.c:405-451) and/or
+    phi (calc_esp_base, math_module.c:453-486).
+    Pinned by: tests/test_oracle_field.py (reference-C fixtures, single-charge
+    closed form, superposition, symmetric-pair cancellation, floor case)."""
+    pts, x, q = _f32(pts, 3), _f32(x, 3), _f32(q)
+    M, N = pts.shape[0], q.size
+    E = np.zeros((M, 3)) if want_E else None
+    phi = np.zeros(M) if want_phi else None
+    lib().oracle_field_f64(M, N, _p(pts), _p(x), _p(q), 1 if soften else 0, _p(E), _p(phi))
+    return E, phi
+
+
+def field_f32(pts, x, q, *, want_E=True, want_phi=False, soften=True):
+    """FP32 port (restated CPU baseline).  Pinned against field_f64 within
+    FP32 rounding in tests/test_oracle_field.py."""
+    pts, x, q = _f32(pts, 3), _f32(x, 3), _f32(q)
+    M, N = pts.shape[0], q.size
+    E = np.zeros((M, 3), np.float32) if want_E else None
+    phi = np.zeros(M, np.float32) if want_phi else None
+    lib().oracle_field_f32(M, N, _p(pts), _p(x), _p(q), 1 if soften else 0, _p(E), _p(phi))
+    return E, phi
+
+
+def topo_f64(starts, n_iter, h, dims, x, q, *, want_points=False):
+    """FP64 streamlines (thread_operation, math_module.c:523-591).
+    Returns feats (L,2), steps (L,), exited (L,) [, pts6 (L,6,3)].
+    Pinned by: tests/test_oracle_topo.py (reference-C fixtures on steps and
+    distance, straight-line closed form with exact step counts, radial-field
+    kappa = 0, curvature identity |u0 x u1|/h, dipole field-line curvature)."""
+    starts, x, q, dims = _f32(starts, 3), _f32(x, 3), _f32(q), _f32(dims)
+    n_iter = np.ascontiguousarray(n_iter, dtype=np.int32).ravel()
+    L, N = starts.shape[0], q.size
+    feats = np.zeros((L, 2))
+    steps = np.zeros(L, np.int32)
+    exited = np.zeros(L, np.int32)
+    pts6 = np.zeros((L, 6, 3)) if want_points else None
+    lib().oracle_topo_f64(N, L, _p(starts), _p(n_iter), float(np.float32(h)), _p(dims),
+                          _p(x), _p(q), _p(feats), _p(steps), _p(exited), _p(pts6))
+    if want_points:
+        return feats, steps, exited, pts6
+    return feats, steps, exited
+
+
+def topo_f32(starts, n_iter, h, dims, x, q):
+    """FP32 port of thread_operation (restated CPU baseline)."""
+    starts, x, q, dims = _f32(starts, 3), _f32(x, 3), _f32(q), _f32(dims)
+    n_iter = np.ascontiguousarray(n_iter, dtype=np.int32).ravel()
+    L, N = starts.shape[0], q.size
+    feats = np.zeros((L, 2), np.float32)
+    steps = np.zeros(L, np.int32)
+    lib().oracle_topo_f32(N, L, _p(starts), _p(n_iter), float(np.float32(h)), _p(dims),
+                          _p(x), _p(q), _p(feats), _p(steps))
+    return feats, steps
+
+
+class Ref:
+    """The reference's own math_module.c (compiled under oracle/_ref).
+
+    Calls follow the reference binding CPET/utils/c_ops.py (argument order,
+    float32 C-order arrays, caller-allocated outputs)."""
+
+    def __init__(self, path: Path = REF_SO):
+        if not Path(path).exists():
+            raise FileNotFoundError(f"{path} not built (needs the reference source)")
+        L = ctypes.CDLL(str(path))
+        L.compute_looped_field.argtypes = [_i, _i, _P, _P, _P, _P]
+        L.calc_esp_base.argtypes = [_P, _P, _i, _P, _P]
+        L.calc_field.argtypes = [_P, _P, _i, _P, _P]
+        L.calc_field_base.argtypes = [_P, _P, _i, _P, _P]
+        L.thread_operation.argtypes = [_i, _i, _f, _P, _P, _P, _P, _P]
+        for fn in (L.compute_looped_field, L.calc_esp_base, L.calc_field,
+                   L.calc_field_base, L.thread_operation):
+            fn.restype = None
+        self.L = L
+
+    @staticmethod
+    def available() -> bool:
+        return REF_SO.exists()
+
+    def compute_looped_field(self, pts, x, q):
+        pts, x, q = _f32(pts, 3), _f32(x, 3), _f32(q)
+        E = np.zeros_like(pts)
+        self.L.compute_looped_field(pts.shape[0], q.size, _p(pts), _p(x), _p(q), _p(E))
+        return E
+
+    def esp_points(self, pts, x, q):
+        """One calc_esp_base call per point, as compute_ESP_on_grid does
+        (CPET/utils/calculator.py:468-471)."""
+        pts, x, q = _f32(pts, 3), _f32(x, 3), _f32(q)
+        out = np.zeros(pts.shape[0], np.float32)
+        one = np.zeros(1, np.float32)
+        for i in range(pts.shape[0]):
+            one[0] = 0.0
+            p = np.ascontiguousarray(pts[i])
+            self.L.calc_esp_base(_p(one), _p(p), q.size, _p(x), _p(q))
+            out[i] = one[0]
+        return out
+
+    def calc_field(self, pt, x, q):
+        pt, x, q = _f32(pt), _f32(x, 3), _f32(q)
+        E = np.zeros(3, np.float32)
+        self.L.calc_field(_p(E), _p(pt), q.size, _p(x), _p(q))
+        return E
+
+    def thread_operation(self, start, n_iter, h, dims, x, q):
+        start, x, q, dims = _f32(start), _f32(x, 3), _f32(q), _f32(dims)
+        ret = np.zeros(2, np.float32)
+        self.L.thread_operation(q.size, int(n_iter), float(np.float32(h)), _p(start), _p(dims),
+                                _p(x), _p(q), _p(ret))
+        return ret
+
+    def topo(self, starts, n_iter, h, dims, x, q):
+        starts = _f32(starts, 3)
+        return np.stack([self.thread_operation(starts[i], n_iter[i], h, dims, x, q)
+                         for i in range(starts.shape[0])])
+
+
+# ---------------------------------------------------------------- numpy parts
+
+def grid_axes(dims, step):
+    """Restates the grid rule: N_cr = int(2 dims / step) in float64
+    (CPET/source/calculator.py:258-259) and inclusive linspace per axis cast
+    to float32 (CPET/utils/calculator.py:218-233).  Pinned by the coordinates
+    of examples/2A_3D-field/outdir/*_efield.dat (11^3, -0.5..0.5 step 0.1) and
+    the int-truncation KAT 2*0.7/0.1 -> 13 (tests/test_oracle_host.py)."""
+    d = np.asarray(dims, dtype=np.float64)
+    n_cr = 2 * d / step
+    return [np.linspace(-d[i], d[i], int(n_cr[i]) + 1).astype(np.float32) for i in range(3)]
+
+
+def histogram2d(a, b, edges_a, edges_b):
+    """Brute-force restatement of numpy.histogramdd binning with explicit
+    edges: bin i holds edges[i] <= v < edges[i+1], the last bin also holds
+    v == edges[-1]; values outside (and NaN) are dropped.  Pinned against
+    numpy.histogram2d on random and edge-valued inputs
+    (tests/test_oracle_host.py)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    ea = np.asarray(edges_a, dtype=np.float64)
+    eb = np.asarray(edges_b, dtype=np.float64)
+    H = np.zeros((ea.size - 1, eb.size - 1), dtype=np.int64)
+
+    def bin_of(v, e):
+        n = e.size - 1
+        if not (e[0] <= v <= e[-1]):
+            return -1
+        if v == e[-1]:
+            return n - 1
+        i = 0
+        while not (e[i] <= v < e[i + 1]):
+            i += 1
+        return i
+
+    for va, vb in zip(a, b):
+        ia, ib = bin_of(va, ea), bin_of(vb, eb)
+        if ia >= 0 and ib >= 0:
+            H[ia, ib] += 1
+    return H
+
+
+def box_basis(center, xpt, ypt):
+    """Box frame T = [x^; y^; z^] with z^ = x^ x y^, y^ = z^ x x^
+    (CPET/utils/calculator.py:200-215)."""
+    x = np.asarray(xpt) - np.asarray(center)
+    y = np.asarray(ypt) - np.asarray(center)
+    xu = x / np.linalg.norm(x)
+    yu = y / np.linalg.norm(y)
+    zu = np.cross(xu, yu)
+    zu = zu / np.linalg.norm(zu)
+    yu = np.cross(zu, xu)
+    yu = yu / np.linalg.norm(yu)
+    return np.column_stack([xu, yu, zu]).T
+
+
+def filter_radius_whole_residue_keep(x, center, radius, resids, resnums):
+    """Keep-mask of filter_radius_whole_residue (CPET/utils/io.py:283-338):
+    runs of consecutive rows with equal (resid, resnum) are kept whole iff any
+    row has |x - center| < radius.  Pinned by the reference test KAT
+    tests/test_filter.py:97-128 (keep WAT#10 only) and hand-built run cases."""
+    xr = np.asarray(x) - np.asarray(center)
+    inside = np.linalg.norm(xr, axis=1) < radius
+    keep = np.zeros(len(inside), dtype=bool)
+    i = 0
+    n = len(inside)
+    while i < n:
+        j = i
+        while j + 1 < n and resids[j + 1] == resids[i] and resnums[j + 1] == resnums[i]:
+            j += 1
+        keep[i:j + 1] = inside[i:j + 1].any()
+        i = j + 1
+    return keep
+
+
+def filter_in_box_keep(x, center, dims):
+    """Keep-mask of filter_in_box (CPET/utils/io.py:422-475): drop rows with
+    every |(x - center)_i| strictly inside dims_i.  Pinned by the reference
+    test KAT tests/test_filter.py:185-210."""
+    xr = np.asarray(x) - np.asarray(center)
+    d = np.asarray(dims)
+    inside = np.ones(xr.shape[0], dtype=bool)
+    for i in range(3):
+        inside &= (xr[:, i] > -d[i]) & (xr[:, i] < d[i])
+    return ~inside
+
+
+def filter_atomspec_drop(ID, filter_dict, intersect):
+    """Indices removed by filter_atomspec (CPET/utils/io.py:341-419).
+    Pinned by the reference test KAT tests/test_filter.py:130-183."""
+    keys = ["atom_number", "atom_type", "resid", "resnum", "chain"]
+    drop = set()
+    if not intersect:
+        for k in filter_dict:
+            vals = filter_dict[k]
+            for j, row in enumerate(ID):
+                if row[keys.index(k)] in vals:
+                    drop.add(j)
+    else:
+        n = len(next(iter(filter_dict.values())))
+        for i in range(n):
+            for j, row in enumerate(ID):
+                ok = True
+                for k in filter_dict:
+                    v = filter_dict[k][i]
+                    if v != "*" and v != "" and row[keys.index(k)] != v:
+                        ok = False
+                        break
+                if ok:
+                    drop.add(j)
+    return sorted(drop)

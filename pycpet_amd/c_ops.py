@@ -1,0 +1,95 @@
+"""Drop-in for the reference's ctypes wrapper class (CPET/utils/c_ops.py).
+
+`Math_ops` keeps the reference's constructor (`shared_loc`) and the method
+names, argument order and return shapes/dtypes of the hot-path methods; the
+native calls go to libcpet_hip.so (gfx950) instead of math_module.so.  Unlike
+the reference, every call checks the returned status and raises CpetError.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native as nat
+from . import ops
+
+
+def _f32(a, last=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a.reshape(-1, last) if last else a.reshape(-1)
+
+
+class Math_ops:
+    def __init__(self, shared_loc=None):
+        # the reference passes the path of its own .so; this class always
+        # binds libcpet_hip.so (shared_loc may name a different build of it)
+        self.math = nat.load(shared_loc) if shared_loc and str(shared_loc).endswith(
+            "libcpet_hip.so") else nat.load()
+
+    def _check(self, rc):
+        nat.check(rc, self.math)
+
+    # CPET/utils/c_ops.py:250-263 -> math_module.c:405-451
+    def compute_looped_field(self, x_0, x, Q):
+        x_0 = _f32(x_0, 3)
+        x = _f32(x, 3)
+        Q = _f32(Q)
+        res = np.zeros_like(x_0, dtype="float32")
+        self._check(self.math.compute_looped_field(
+            int(x_0.shape[0]), len(Q), x_0.ctypes.data, x.ctypes.data, Q.ctypes.data,
+            res.ctypes.data))
+        return res
+
+    # CPET/utils/c_ops.py:326-341 -> math_module.c:453-486
+    def calc_esp_base(self, x_0, x, Q):
+        x_0 = _f32(x_0)
+        x = _f32(x, 3)
+        Q = _f32(Q)
+        res = np.zeros(1, dtype="float32")
+        self._check(self.math.calc_esp_base(res.ctypes.data, x_0.ctypes.data, len(Q),
+                                            x.ctypes.data, Q.ctypes.data))
+        return res
+
+    # CPET/utils/c_ops.py:343-358 -> math_module.c:296-333
+    def calc_field_base(self, x_0, x, Q):
+        x_0 = _f32(x_0)
+        x = _f32(x, 3)
+        Q = _f32(Q)
+        res = np.zeros(3, dtype="float32")
+        self._check(self.math.calc_field_base(res.ctypes.data, x_0.ctypes.data, len(Q),
+                                              x.ctypes.data, Q.ctypes.data))
+        return res
+
+    # CPET/utils/c_ops.py:360-375 -> math_module.c:255-293
+    def calc_field(self, x_0, x, Q):
+        x_0 = _f32(x_0)
+        x = _f32(x, 3)
+        Q = _f32(Q)
+        res = np.zeros(3, dtype="float32")
+        self._check(self.math.calc_field(res.ctypes.data, x_0.ctypes.data, len(Q),
+                                         x.ctypes.data, Q.ctypes.data))
+        return res
+
+    # CPET/utils/c_ops.py:281-302 -> math_module.c:523-591
+    def thread_operation(self, x_0, n_iter, x, Q, step_size, dimensions):
+        x_0 = _f32(x_0)
+        x = _f32(x, 3)
+        Q = _f32(Q)
+        dims = _f32(dimensions)
+        res = np.zeros(2, dtype="float32")
+        self._check(self.math.thread_operation(
+            len(Q), int(n_iter), ctypes.c_float(float(step_size)), x_0.ctypes.data,
+            dims.ctypes.data, x.ctypes.data, Q.ctypes.data, res.ctypes.data))
+        return res
+
+    # Batched forms (no per-point / per-streamline FFI loop).
+    def esp_points(self, x_0, x, Q):
+        """phi at every row of x_0 in one launch (float32)."""
+        _, phi = ops.field_points(_f32(x_0, 3), x, Q, want_E=False, want_phi=True)
+        return phi
+
+    def thread_operation_batch(self, starts, n_iter, x, Q, step_size, dimensions):
+        """thread_operation for every start in one launch; (L,2) float32."""
+        r = ops.topo(starts, n_iter, step_size, dimensions, x, Q)
+        return r.feats.astype(np.float32)

@@ -1,0 +1,535 @@
+// C-ABI layer of libcpet_hip.so: contexts, staging, argument validation and
+// the reference-compatible entry points (names and argument meaning of
+// CPET/utils/math_module.c, bound by CPET/utils/c_ops.py).
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "cpet_internal.h"
+
+namespace cpet {
+
+// Implemented in field.hip / topo.hip.
+int field_launch(cpet_ctx* ctx, bool grid_mode, int nx, int ny, int nz,
+                 const float* ax, const float* ay, const float* az, long long M,
+                 const float* pts, const float4* ch, int npad, void* E_out,
+                 void* phi_out, int soften, uint32_t flags, int forced_k,
+                 int forced_splits);
+int topo_run(cpet_ctx* ctx, int N, int L, const float* starts,
+             const int32_t* n_iter, float h, const float* dims, const float4* ch,
+             int npad, double* feats, int32_t* steps, int32_t* exited, int nb_d,
+             const double* edges_d, int nb_c, const double* edges_c,
+             uint32_t* hist, uint32_t flags);
+
+namespace {
+thread_local char g_err[512] = "";
+std::mutex g_default_mu;
+cpet_ctx* g_default = nullptr;
+
+__global__ __launch_bounds__(256) void pack_charges_kernel(
+    int N, int npad, const float* __restrict__ x, const float* __restrict__ q,
+    float4* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= npad) return;
+  if (j < N)
+    out[j] = make_float4(x[3 * j], x[3 * j + 1], x[3 * j + 2], q[j]);
+  else
+    out[j] = make_float4(kPadCoord, kPadCoord, kPadCoord, 0.f);
+}
+
+bool finite3(const float* v) {
+  return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]);
+}
+}  // namespace
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+int ctx_activate(cpet_ctx* ctx) {
+  CPET_HIP_TRY(hipSetDevice(ctx->device));
+  return CPET_OK;
+}
+
+int ctx_buf(cpet_ctx* ctx, int slot, size_t bytes, void** out) {
+  if (slot < 0 || slot >= kSlotCount) {
+    set_error("internal: bad scratch slot %d", slot);
+    return CPET_ERR_ARG;
+  }
+  DevBuf& b = ctx->bufs[slot];
+  if (bytes == 0) bytes = 256;
+  if (b.bytes < bytes) {
+    if (b.ptr) {
+      // the old buffer may still be read by queued work
+      CPET_HIP_TRY(hipStreamSynchronize(ctx->stream));
+      CPET_HIP_TRY(hipFree(b.ptr));
+      b.ptr = nullptr;
+      b.bytes = 0;
+    }
+    size_t want = bytes + bytes / 4;  // grow with headroom
+    want = (want + 4095) & ~(size_t)4095;
+    hipError_t e = hipMalloc(&b.ptr, want);
+    if (e != hipSuccess) {
+      b.ptr = nullptr;
+      set_error("hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+      return CPET_ERR_NOMEM;
+    }
+    b.bytes = want;
+  }
+  *out = b.ptr;
+  return CPET_OK;
+}
+
+int stage_in(cpet_ctx* ctx, int slot, const void* src, size_t bytes, bool dev,
+             const void** out) {
+  if (dev) {
+    *out = src;
+    return CPET_OK;
+  }
+  void* d = nullptr;
+  int rc = ctx_buf(ctx, slot, bytes, &d);
+  if (rc) return rc;
+  if (bytes)
+    CPET_HIP_TRY(hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice,
+                                ctx->stream));
+  *out = d;
+  return CPET_OK;
+}
+
+int pack_charges(cpet_ctx* ctx, int N, const float* x, const float* q,
+                 bool dev, const float4** out, int* npad) {
+  const int np = round_up(N > 0 ? N : 1, kTile);
+  const void* dx = nullptr;
+  const void* dq = nullptr;
+  int rc = stage_in(ctx, kSlotX, x, (size_t)N * 3 * sizeof(float), dev, &dx);
+  if (rc) return rc;
+  rc = stage_in(ctx, kSlotQ, q, (size_t)N * sizeof(float), dev, &dq);
+  if (rc) return rc;
+  void* d = nullptr;
+  rc = ctx_buf(ctx, kSlotCharges, (size_t)np * sizeof(float4), &d);
+  if (rc) return rc;
+  hipLaunchKernelGGL(pack_charges_kernel, dim3((np + 255) / 256), dim3(256), 0,
+                     ctx->stream, N, np, (const float*)dx, (const float*)dq,
+                     (float4*)d);
+  CPET_HIP_TRY(hipGetLastError());
+  *out = (const float4*)d;
+  *npad = np;
+  return CPET_OK;
+}
+
+int default_ctx(cpet_ctx** out) {
+  std::lock_guard<std::mutex> lk(g_default_mu);
+  if (!g_default) {
+    int dev = 0;
+    if (const char* s = std::getenv("CPET_DEVICE")) dev = std::atoi(s);
+    cpet_ctx* c = nullptr;
+    int rc = cpet_ctx_create(dev, &c);
+    if (rc) return rc;
+    g_default = c;
+  }
+  *out = g_default;
+  return CPET_OK;
+}
+
+int prof_mark(cpet_ctx* ctx, int i) {
+  if (!ctx->profile) return CPET_OK;
+  if (!ctx->ev[i]) CPET_HIP_TRY(hipEventCreate(&ctx->ev[i]));
+  CPET_HIP_TRY(hipEventRecord(ctx->ev[i], ctx->stream));
+  ctx->marks = i + 1;
+  return CPET_OK;
+}
+
+int prof_collect(cpet_ctx* ctx, int n) {
+  if (!ctx->profile || n < 2) return CPET_OK;
+  CPET_HIP_TRY(hipEventSynchronize(ctx->ev[n - 1]));
+  for (int i = 0; i + 1 < n && i < 4; ++i)
+    CPET_HIP_TRY(hipEventElapsedTime(&ctx->phase_ms[i], ctx->ev[i], ctx->ev[i + 1]));
+  ctx->marks = 0;
+  return CPET_OK;
+}
+
+// Copies device results back for host-pointer calls.
+static int copy_out(cpet_ctx* ctx, void* host, const void* dev, size_t bytes) {
+  if (bytes)
+    CPET_HIP_TRY(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost,
+                                ctx->stream));
+  return CPET_OK;
+}
+
+}  // namespace cpet
+
+using namespace cpet;
+
+extern "C" {
+
+const char* cpet_last_error(void) { return g_err; }
+
+int cpet_device_count(int* n) {
+  if (!n) {
+    set_error("n is NULL");
+    return CPET_ERR_ARG;
+  }
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  *n = (e == hipSuccess) ? c : 0;
+  return CPET_OK;
+}
+
+int cpet_ctx_create(int device, cpet_ctx** out) {
+  if (!out) {
+    set_error("out is NULL");
+    return CPET_ERR_ARG;
+  }
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    set_error("no GPU visible (hipGetDeviceCount returned %d devices)", n);
+    return CPET_ERR_NODEV;
+  }
+  if (device < 0 || device >= n) {
+    set_error("device %d out of range [0,%d)", device, n);
+    return CPET_ERR_ARG;
+  }
+  cpet_ctx* c = new cpet_ctx();
+  c->device = device;
+  c->bufs.resize(kSlotCount);
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess)
+    e = hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount,
+                              device);
+  if (e != hipSuccess) {
+    set_error("context setup failed: %s", hipGetErrorString(e));
+    delete c;
+    return CPET_ERR_HIP;
+  }
+  c->own_stream = true;
+  c->own = c->stream;
+  *out = c;
+  return CPET_OK;
+}
+
+int cpet_ctx_destroy(cpet_ctx* ctx) {
+  if (!ctx) return CPET_OK;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    if (ctx->own_stream && ctx->own)
+      (void)hipStreamSynchronize(ctx->own);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto& b : ctx->bufs)
+      if (b.ptr) (void)hipFree(b.ptr);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    for (auto& ev : ctx->ev)
+      if (ev) (void)hipEventDestroy(ev);
+    if (ctx->own) (void)hipStreamDestroy(ctx->own);
+  }
+  if (ctx == g_default) {
+    std::lock_guard<std::mutex> lk(g_default_mu);
+    g_default = nullptr;
+  }
+  delete ctx;
+  return CPET_OK;
+}
+
+int cpet_ctx_set_stream(cpet_ctx* ctx, void* hip_stream) {
+  if (!ctx) {
+    set_error("ctx is NULL");
+    return CPET_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->stream = hip_stream ? (hipStream_t)hip_stream : ctx->own;
+  return CPET_OK;
+}
+
+int cpet_ctx_get_stream(cpet_ctx* ctx, void** hip_stream) {
+  if (!ctx || !hip_stream) {
+    set_error("NULL argument");
+    return CPET_ERR_ARG;
+  }
+  *hip_stream = (void*)ctx->stream;
+  return CPET_OK;
+}
+
+int cpet_ctx_sync(cpet_ctx* ctx) {
+  if (!ctx) {
+    set_error("ctx is NULL");
+    return CPET_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  int rc = ctx_activate(ctx);
+  if (rc) return rc;
+  CPET_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return CPET_OK;
+}
+
+int cpet_ctx_set_profiling(cpet_ctx* ctx, int on) {
+  if (!ctx) {
+    set_error("ctx is NULL");
+    return CPET_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->profile = on != 0;
+  return CPET_OK;
+}
+
+int cpet_ctx_phase_times(cpet_ctx* ctx, float* ms, int n) {
+  if (!ctx || !ms || n < 0) {
+    set_error("cpet_ctx_phase_times: bad arguments");
+    return CPET_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (ctx->marks > 1) {  // a CPET_F_NO_SYNC call left marks to collect
+    int rc = ctx_activate(ctx);
+    if (rc) return rc;
+    if ((rc = prof_collect(ctx, ctx->marks))) return rc;
+  }
+  for (int i = 0; i < n && i < 4; ++i) ms[i] = ctx->phase_ms[i];
+  return CPET_OK;
+}
+
+int cpet_set_tuning(cpet_ctx* ctx, int kind, int a, int b) {
+  if (!ctx || kind < 0 || kind > 2 || a < 0 || b < 0) {
+    set_error("cpet_set_tuning: bad arguments");
+    return CPET_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->tune[kind][0] = a;
+  ctx->tune[kind][1] = b;
+  return CPET_OK;
+}
+
+static int field_common(cpet_ctx* ctx, bool grid_mode, int nx, int ny, int nz,
+                        const float* ax, const float* ay, const float* az,
+                        long long M, const float* pts, int N, const float* x,
+                        const float* q, void* E_out, void* phi_out, int soften,
+                        uint32_t flags) {
+  if (!ctx) {
+    set_error("ctx is NULL");
+    return CPET_ERR_ARG;
+  }
+  if (M <= 0 || N <= 0) {
+    set_error("need M > 0 and N > 0 (got M=%lld N=%d)", M, N);
+    return CPET_ERR_ARG;
+  }
+  if (!x || !q || (!E_out && !phi_out) ||
+      (grid_mode ? (!ax || !ay || !az) : !pts)) {
+    set_error("NULL array argument");
+    return CPET_ERR_ARG;
+  }
+  const bool dev = (flags & CPET_F_DEVICE) != 0;
+  const size_t osz = (flags & CPET_F_OUT_F64) ? sizeof(double) : sizeof(float);
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  int rc = ctx_activate(ctx);
+  if (rc) return rc;
+  const float4* ch = nullptr;
+  int npad = 0;
+  rc = pack_charges(ctx, N, x, q, dev, &ch, &npad);
+  if (rc) return rc;
+  const void *dax = nullptr, *day = nullptr, *daz = nullptr, *dpts = nullptr;
+  if (grid_mode) {
+    void* axes = nullptr;
+    if (dev) {
+      dax = ax;
+      day = ay;
+      daz = az;
+    } else {
+      rc = ctx_buf(ctx, kSlotAxes, (size_t)(nx + ny + nz) * sizeof(float), &axes);
+      if (rc) return rc;
+      float* a = (float*)axes;
+      CPET_HIP_TRY(hipMemcpyAsync(a, ax, nx * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+      CPET_HIP_TRY(hipMemcpyAsync(a + nx, ay, ny * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+      CPET_HIP_TRY(hipMemcpyAsync(a + nx + ny, az, nz * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+      dax = a;
+      day = a + nx;
+      daz = a + nx + ny;
+    }
+  } else {
+    rc = stage_in(ctx, kSlotPts, pts, (size_t)M * 3 * sizeof(float), dev, &dpts);
+    if (rc) return rc;
+  }
+  void* dE = E_out;
+  void* dphi = phi_out;
+  if (!dev) {
+    if (E_out) {
+      rc = ctx_buf(ctx, kSlotOut0, (size_t)M * 3 * osz, &dE);
+      if (rc) return rc;
+    }
+    if (phi_out) {
+      rc = ctx_buf(ctx, kSlotOut1, (size_t)M * osz, &dphi);
+      if (rc) return rc;
+    }
+  }
+  const int kind = grid_mode ? 0 : 1;
+  rc = field_launch(ctx, grid_mode, nx, ny, nz, (const float*)dax,
+                    (const float*)day, (const float*)daz, M,
+                    (const float*)dpts, ch, npad, E_out ? dE : nullptr,
+                    phi_out ? dphi : nullptr, soften, flags, ctx->tune[kind][0],
+                    ctx->tune[kind][1]);
+  if (rc) return rc;
+  if (!dev) {
+    if (E_out && (rc = copy_out(ctx, E_out, dE, (size_t)M * 3 * osz))) return rc;
+    if (phi_out && (rc = copy_out(ctx, phi_out, dphi, (size_t)M * osz))) return rc;
+  }
+  if (!dev || !(flags & CPET_F_NO_SYNC)) {
+    CPET_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if ((rc = prof_collect(ctx, 3))) return rc;
+  }
+  return CPET_OK;
+}
+
+int cpet_field_points(cpet_ctx* ctx, int M, int N, const float* pts,
+                      const float* x, const float* q, void* E_out,
+                      void* phi_out, int soften, uint32_t flags) {
+  return field_common(ctx, false, 0, 0, 0, nullptr, nullptr, nullptr, M, pts,
+                      N, x, q, E_out, phi_out, soften, flags);
+}
+
+int cpet_field_grid(cpet_ctx* ctx, int nx, int ny, int nz, const float* ax,
+                    const float* ay, const float* az, int N, const float* x,
+                    const float* q, void* E_out, void* phi_out, int soften,
+                    uint32_t flags) {
+  if (nx <= 0 || ny <= 0 || nz <= 0) {
+    set_error("grid dimensions must be positive (%d,%d,%d)", nx, ny, nz);
+    return CPET_ERR_ARG;
+  }
+  const long long M = (long long)nx * ny * nz;
+  return field_common(ctx, true, nx, ny, nz, ax, ay, az, M, nullptr, N, x, q,
+                      E_out, phi_out, soften, flags);
+}
+
+int cpet_topo(cpet_ctx* ctx, int N, int L, const float* starts,
+              const int32_t* n_iter, float h, const float* dims,
+              const float* x, const float* q, double* feats, int32_t* steps,
+              int32_t* exited, int nb_d, const double* edges_d, int nb_c,
+              const double* edges_c, uint32_t* hist, uint32_t flags) {
+  if (!ctx) {
+    set_error("ctx is NULL");
+    return CPET_ERR_ARG;
+  }
+  if (N <= 0 || L <= 0) {
+    set_error("need N > 0 and L > 0 (got N=%d L=%d)", N, L);
+    return CPET_ERR_ARG;
+  }
+  if (!starts || !n_iter || !dims || !x || !q || !feats) {
+    set_error("NULL array argument");
+    return CPET_ERR_ARG;
+  }
+  if (!(std::isfinite(h) && h > 0.f)) {
+    set_error("step size must be finite and > 0 (got %g)", (double)h);
+    return CPET_ERR_ARG;
+  }
+  if (hist && (nb_d <= 0 || nb_c <= 0 || !edges_d || !edges_c)) {
+    set_error("histogram requested without valid bins/edges");
+    return CPET_ERR_ARG;
+  }
+  const bool dev = (flags & CPET_F_DEVICE) != 0;
+  float hdims[3];
+  if (dev) {
+    // dims is tiny; read it back so the kernel gets plain values
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    int rc = ctx_activate(ctx);
+    if (rc) return rc;
+    CPET_HIP_TRY(hipMemcpyAsync(hdims, dims, sizeof(hdims), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    CPET_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  } else {
+    std::memcpy(hdims, dims, sizeof(hdims));
+  }
+  if (!finite3(hdims) || !(hdims[0] > 0.f && hdims[1] > 0.f && hdims[2] > 0.f)) {
+    set_error("box half-widths must be finite and > 0");
+    return CPET_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  int rc = ctx_activate(ctx);
+  if (rc) return rc;
+  const float4* ch = nullptr;
+  int npad = 0;
+  rc = pack_charges(ctx, N, x, q, dev, &ch, &npad);
+  if (rc) return rc;
+  return topo_run(ctx, N, L, starts, n_iter, h, hdims, ch, npad, feats, steps,
+                  exited, nb_d, edges_d, nb_c, edges_c, hist, flags);
+}
+
+// ---------------------------------------------------------------------------
+// Reference-compatible entry points (host arrays, default context).
+
+int compute_looped_field(int total_points, int n_charges, const float* x_0,
+                         const float* x, const float* Q, float* E) {
+  cpet_ctx* c = nullptr;
+  int rc = default_ctx(&c);
+  if (rc) return rc;
+  return cpet_field_points(c, total_points, n_charges, x_0, x, Q, E, nullptr,
+                           1, 0u);
+}
+
+int calc_esp_base(float* ESP, const float* x_init, int n_charges,
+                  const float* x, const float* Q) {
+  if (!ESP) {
+    set_error("ESP is NULL");
+    return CPET_ERR_ARG;
+  }
+  cpet_ctx* c = nullptr;
+  int rc = default_ctx(&c);
+  if (rc) return rc;
+  float phi = 0.f;
+  rc = cpet_field_points(c, 1, n_charges, x_init, x, Q, nullptr, &phi, 0, 0u);
+  if (rc) return rc;
+  ESP[0] += phi;  // the reference accumulates into ESP[0]
+  return CPET_OK;
+}
+
+int calc_field(float* E, const float* x_init, int n_charges, const float* x,
+               const float* Q) {
+  if (!E) {
+    set_error("E is NULL");
+    return CPET_ERR_ARG;
+  }
+  cpet_ctx* c = nullptr;
+  int rc = default_ctx(&c);
+  if (rc) return rc;
+  return cpet_field_points(c, 1, n_charges, x_init, x, Q, E, nullptr, 0, 0u);
+}
+
+int calc_field_base(float* E, const float* x_init, int n_charges,
+                    const float* x, const float* Q) {
+  if (!E) {
+    set_error("E is NULL");
+    return CPET_ERR_ARG;
+  }
+  float tmp[3] = {0.f, 0.f, 0.f};
+  int rc = calc_field(tmp, x_init, n_charges, x, Q);
+  if (rc) return rc;
+  E[0] += tmp[0];  // the reference accumulates into E
+  E[1] += tmp[1];
+  E[2] += tmp[2];
+  return CPET_OK;
+}
+
+int thread_operation(int n_charges, int n_iter, float step_size,
+                     const float* x_0, const float* dimensions, const float* x,
+                     const float* Q, float* ret) {
+  if (!ret) {
+    set_error("ret is NULL");
+    return CPET_ERR_ARG;
+  }
+  cpet_ctx* c = nullptr;
+  int rc = default_ctx(&c);
+  if (rc) return rc;
+  double f[2];
+  const int32_t it = n_iter;
+  rc = cpet_topo(c, n_charges, 1, x_0, &it, step_size, dimensions, x, Q, f,
+                 nullptr, nullptr, 0, nullptr, 0, nullptr, nullptr, 0u);
+  if (rc) return rc;
+  ret[0] = (float)f[0];
+  ret[1] = (float)f[1];
+  return CPET_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,128 @@
+// Internal declarations shared by the HIP translation units of libcpet_hip.so.
+// Nothing here is part of the C-ABI; the public surface is include/cpet_hip.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/cpet_hip.h"
+
+namespace cpet {
+
+// Coulomb constant in eV*A/e^2, as used by every reference kernel
+// (CPET/utils/math_module.c:305,412,462).
+constexpr double kCoulomb = 14.3996451;
+
+// Charges are padded to a multiple of kTile with zero charges placed far away
+// (their terms underflow to exactly 0), so kernels never handle a remainder.
+constexpr int kTile = 64;
+constexpr float kPadCoord = 1.0e18f;
+
+// Precision modes of the FP32 field kernels (bits 8..9 of the flags word).
+enum Prec : int {
+  kPrecTiled = 0,  // FP32 per 64-charge tile, FP64 across tiles (default)
+  kPrecNaive = 1,  // one FP32 running sum over all charges
+  kPrecKahan = 2,  // Kahan-compensated FP32 inside a tile, FP64 across tiles
+  kPrecF64 = 3,    // everything in FP64 (diagnostics)
+};
+
+// What a field launch computes.
+enum What : int {
+  kWhatESoft = 0,     // E with the r^2 >= 1e-6 floor (volume)
+  kWhatPhi = 1,       // phi only, no floor (volume_ESP)
+  kWhatESoftPhi = 2,  // both, E floored and phi not
+  kWhatEHard = 3,     // E without floor (point field)
+};
+
+void set_error(const char* fmt, ...);
+
+#define CPET_HIP_TRY(expr)                                                   \
+  do {                                                                       \
+    hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess) {                                                  \
+      ::cpet::set_error("%s failed: %s (%s:%d)", #expr,                      \
+                        hipGetErrorString(e_), __FILE__, __LINE__);          \
+      return CPET_ERR_HIP;                                                   \
+    }                                                                        \
+  } while (0)
+
+// A grow-only device buffer owned by a context.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace cpet
+
+// Context: one device, one stream, a set of scratch buffers.  Calls on one
+// context are serialised by its mutex; different contexts run concurrently.
+struct cpet_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;  // stream work is queued on
+  hipStream_t own = nullptr;     // the context's own stream
+  bool own_stream = false;
+  int num_cus = 0;
+  int tune[3][2] = {{0, 0}, {0, 0}, {0, 0}};  // autotuner overrides
+  std::mutex mu;
+  std::vector<cpet::DevBuf> bufs;  // indexed by Slot
+  // host-side pinned staging for small results
+  void* pinned = nullptr;
+  size_t pinned_bytes = 0;
+  // opt-in per-phase timing with HIP events on the context's stream
+  bool profile = false;
+  hipEvent_t ev[8] = {};
+  float phase_ms[4] = {0.f, 0.f, 0.f, 0.f};
+  int marks = 0;  // events recorded by the last call, not yet collected
+};
+
+namespace cpet {
+
+enum Slot : int {
+  kSlotCharges = 0,  // packed float4 charges
+  kSlotPts,          // staged points / starts
+  kSlotAxes,         // grid axes
+  kSlotOut0,         // staged outputs
+  kSlotOut1,
+  kSlotPartial,      // split partial sums
+  kSlotX,            // staged raw x
+  kSlotQ,            // staged raw q
+  kSlotTopoState,    // final points, steps, flags
+  kSlotTopoOrder,    // work order
+  kSlotTopoMisc,     // counters, histogram, edges
+  kSlotIter,         // staged n_iter
+  kSlotOut2,
+  kSlotFrame0,       // frame-transform scratch
+  kSlotFrame1,
+  kSlotFrame2,
+  kSlotCount
+};
+
+// Returns a device buffer of at least `bytes` for `slot`, growing it if needed.
+int ctx_buf(cpet_ctx* ctx, int slot, size_t bytes, void** out);
+// Makes ctx->device current on the calling thread.
+int ctx_activate(cpet_ctx* ctx);
+// Copies host->device (or passes a device pointer through when dev != 0).
+int stage_in(cpet_ctx* ctx, int slot, const void* src, size_t bytes, bool dev,
+             const void** out);
+// Global default context for the reference-compatible entry points.
+int default_ctx(cpet_ctx** out);
+
+// Packs x (N,3) and q (N) into padded float4 charges; returns the padded
+// count through npad.  x/q may be host or device pointers (dev flag).
+int pack_charges(cpet_ctx* ctx, int N, const float* x, const float* q,
+                 bool dev, const float4** out, int* npad);
+
+inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
+
+// Phase timing: mark(ctx, i) records event i when profiling is on;
+// collect(ctx, n) (after the stream has been synchronised) stores the n-1
+// intervals between consecutive marks in ctx->phase_ms.
+int prof_mark(cpet_ctx* ctx, int i);
+int prof_collect(cpet_ctx* ctx, int n);
+
+}  // namespace cpet

@@ -1,0 +1,504 @@
+// Streamline ("topology") kernels for gfx950.
+//
+// Replaces thread_operation (CPET/utils/math_module.c:523-591) fanned over a
+// multiprocessing.Pool (CPET/source/calculator.py:675-712), and the torch
+// topo_GPU path (CPET/source/calculator.py:793-978, CPET/utils/gpu.py).
+//
+// Semantics (per streamline, same as the reference):
+//   x <- x + h E(x)/|E(x)| for up to n_iter steps (Euler on the unit field
+//   direction, math_module.c:489-503), stopping after the first point that
+//   is strictly outside +-dims (math_module.c:539-547; that point is the final
+//   point xf); then x0+, x0++, xf+, xf++ (math_module.c:565-568),
+//   kappa = |v' x v''| / |v'|^3 with v' = x+ - x, v'' = x++ - 2x+ + x
+//   (math_module.c:108-121, 575-584), feature = [|x0 - xf|, mean kappa].
+//
+// Pipeline:
+//   1. order: counting sort of streamline ids by n_iter, longest first, so
+//      the persistent kernel's tail consists of short streamlines.
+//   2. propagate (persistent): a workgroup of S waves owns 64 streamline
+//      slots; wave s sums the field of charge slice s for all 64 slots (FP32
+//      per 64-charge tile, FP64 across tiles), the S partials are reduced in
+//      a fixed order through LDS and every wave applies the identical FP64
+//      position update.  A finished slot takes the next id from a global
+//      queue.  Results do not depend on which slot runs a streamline.
+//   3. epilogue: one lane per streamline, the four curvature-defining steps
+//      in FP64 (FP32 finite differences at h = 1e-3 lose most digits),
+//      distance, and the 2-D histogram with numpy.histogramdd bin rules.
+
+#include <cmath>
+
+#include "cpet_internal.h"
+
+namespace cpet {
+namespace {
+
+constexpr int kNumBuckets = 65536;
+
+__device__ __forceinline__ double rsqrt_f64(double x) {
+  // Hardware estimate refined by two Newton-Raphson steps for 1/sqrt(x).
+  double y = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  double y1 = y * fma(-hx * y, y, 1.5);
+  y1 = y1 * fma(-hx * y1, y1, 1.5);
+  return (x == 0.0) ? y : y1;
+}
+
+__global__ __launch_bounds__(256) void fill_u32_kernel(unsigned int* p,
+                                                       long long n,
+                                                       unsigned int v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---- 1. longest-first order -------------------------------------------
+
+__device__ __forceinline__ int bucket_of(int n) {
+  const int k = n < 0 ? 0 : (n >= kNumBuckets ? kNumBuckets - 1 : n);
+  return kNumBuckets - 1 - k;  // descending n_iter
+}
+
+__global__ __launch_bounds__(256) void bucket_count_kernel(
+    int L, const int32_t* __restrict__ n_iter, unsigned int* counts) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l < L) atomicAdd(&counts[bucket_of(n_iter[l])], 1u);
+}
+
+// Exclusive scan of kNumBuckets counts by one 1024-thread block.
+__global__ __launch_bounds__(1024) void bucket_scan_kernel(unsigned int* counts) {
+  __shared__ unsigned int sums[1024];
+  constexpr int per = kNumBuckets / 1024;
+  const int t = threadIdx.x;
+  unsigned int local = 0;
+  for (int i = 0; i < per; ++i) local += counts[t * per + i];
+  sums[t] = local;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const unsigned int v = t >= off ? sums[t - off] : 0u;
+    __syncthreads();
+    sums[t] += v;
+    __syncthreads();
+  }
+  unsigned int run = sums[t] - local;  // exclusive prefix of this chunk
+  for (int i = 0; i < per; ++i) {
+    const unsigned int c = counts[t * per + i];
+    counts[t * per + i] = run;
+    run += c;
+  }
+}
+
+__global__ __launch_bounds__(256) void bucket_scatter_kernel(
+    int L, const int32_t* __restrict__ n_iter, unsigned int* offsets,
+    int32_t* __restrict__ order) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l < L) {
+    const unsigned int pos = atomicAdd(&offsets[bucket_of(n_iter[l])], 1u);
+    order[pos] = l;
+  }
+}
+
+// ---- 2. persistent propagation ----------------------------------------
+
+struct PropArgs {
+  int npad;
+  int L;
+  const float* starts;
+  const int32_t* n_iter;
+  const int32_t* order;
+  double h, d0, d1, d2;
+  double* fin;      // (L,3) final points
+  int32_t* steps;   // (L)
+  int32_t* exited;  // (L)
+  unsigned int* counter;
+};
+
+// Unsoftened field of charge tiles [t0, t1) at one point, FP32 per tile and
+// FP64 across tiles (the k factor is applied by the caller).
+__device__ __forceinline__ void slice_field(float px, float py, float pz,
+                                            const float4* __restrict__ ch,
+                                            int t0, int t1, double& ex,
+                                            double& ey, double& ez) {
+  ex = ey = ez = 0.0;
+  for (int t = t0; t < t1; ++t) {
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    const float4* cp = ch + t * kTile;
+#pragma unroll 16
+    for (int j = 0; j < kTile; ++j) {
+      const float4 c = cp[j];
+      const float dx = px - c.x, dy = py - c.y, dz = pz - c.z;
+      const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+      const float ri = __builtin_amdgcn_rsqf(r2);
+      const float s = (c.w * ri) * (ri * ri);
+      fx = fmaf(s, dx, fx);
+      fy = fmaf(s, dy, fy);
+      fz = fmaf(s, dz, fz);
+    }
+    ex += (double)fx;
+    ey += (double)fy;
+    ez += (double)fz;
+  }
+}
+
+enum SlotState : int { kActive = 0, kNeed = 1, kDone = 2 };
+
+// The charge pointer is a separate __restrict__ parameter so the compiler
+// may read charges through the scalar cache despite the kernel's stores.
+template <int S>
+__global__ __launch_bounds__(64 * S) void topo_propagate_kernel(
+    const float4* __restrict__ ch, PropArgs a) {
+  __shared__ double part[2][S][3][64];
+  __shared__ int nsid[2][64];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int ntiles = a.npad / kTile;
+  const int tpw = (ntiles + S - 1) / S;
+  const int t0 = min(wave * tpw, ntiles);
+  const int t1 = min(t0 + tpw, ntiles);
+
+  // Slot state; identical in every wave of the workgroup.
+  int st = kDone, sid = 0, step = 0, nmax = 0;
+  double px = 0.0, py = 0.0, pz = 0.0;
+
+  auto finish = [&](int out) {
+    if (wave == 0) {
+      a.fin[3 * sid] = px;
+      a.fin[3 * sid + 1] = py;
+      a.fin[3 * sid + 2] = pz;
+      a.steps[sid] = step;
+      a.exited[sid] = out;
+    }
+    st = kNeed;
+  };
+  auto load = [&](int q) {
+    if (q >= a.L) {
+      st = kDone;
+      return;
+    }
+    sid = a.order[q];
+    px = (double)a.starts[3 * sid];
+    py = (double)a.starts[3 * sid + 1];
+    pz = (double)a.starts[3 * sid + 2];
+    nmax = a.n_iter[sid];
+    step = 0;
+    if (nmax <= 0)
+      finish(0);  // the reference loop body never runs: xf = x0
+    else
+      st = kActive;
+  };
+
+  load(blockIdx.x * 64 + lane);
+  int par = 0;
+  while (__ballot(st != kDone) != 0ull) {
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if (st == kActive)
+      slice_field((float)px, (float)py, (float)pz, ch, t0, t1, fx, fy, fz);
+    part[par][wave][0][lane] = fx;
+    part[par][wave][1][lane] = fy;
+    part[par][wave][2][lane] = fz;
+    if (wave == 0 && st == kNeed)
+      nsid[par][lane] = (int)atomicAdd(a.counter, 1u);
+    __syncthreads();
+    if (st == kActive) {
+      double ex = 0.0, ey = 0.0, ez = 0.0;
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        ex += part[par][s][0][lane];
+        ey += part[par][s][1][lane];
+        ez += part[par][s][2][lane];
+      }
+      ex *= kCoulomb;
+      ey *= kCoulomb;
+      ez *= kCoulomb;
+      const double n = sqrt(ex * ex + ey * ey + ez * ez);
+      px = px + a.h * ex / n;
+      py = py + a.h * ey / n;
+      pz = pz + a.h * ez / n;
+      ++step;
+      const int out = (px < -a.d0 || px > a.d0 || py < -a.d1 || py > a.d1 ||
+                       pz < -a.d2 || pz > a.d2)
+                          ? 1
+                          : 0;
+      if (out || step >= nmax) finish(out);
+    } else if (st == kNeed) {
+      load(nsid[par][lane]);
+    }
+    par ^= 1;
+  }
+}
+
+// ---- 3. FP64 epilogue -------------------------------------------------
+
+struct EpiArgs {
+  int L, npad;
+  const float* starts;
+  const double* fin;
+  double h;
+  double* feats;
+  const double* ed;
+  int nbd;
+  const double* ec;
+  int nbc;
+  unsigned int* hist;
+  int lds_hist;
+};
+
+// One Euler step for two points at once, FP64 throughout.
+__device__ __forceinline__ void step2_f64(const double (&p)[2][3],
+                                          double (&o)[2][3],
+                                          const float4* __restrict__ ch,
+                                          int npad, double h) {
+  double e[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+  for (int j = 0; j < npad; ++j) {
+    const float4 c = ch[j];
+    const double cx = c.x, cy = c.y, cz = c.z, q = c.w;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const double dx = p[k][0] - cx, dy = p[k][1] - cy, dz = p[k][2] - cz;
+      const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+      const double ri = rsqrt_f64(r2);
+      const double s = (q * ri) * (ri * ri);
+      e[k][0] = fma(s, dx, e[k][0]);
+      e[k][1] = fma(s, dy, e[k][1]);
+      e[k][2] = fma(s, dz, e[k][2]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const double ex = kCoulomb * e[k][0], ey = kCoulomb * e[k][1],
+                 ez = kCoulomb * e[k][2];
+    const double n = sqrt(ex * ex + ey * ey + ez * ez);
+    o[k][0] = p[k][0] + h * ex / n;
+    o[k][1] = p[k][1] + h * ey / n;
+    o[k][2] = p[k][2] + h * ez / n;
+  }
+}
+
+__device__ __forceinline__ double curve_f64(const double (&a)[3],
+                                            const double (&b)[3]) {
+  const double c0 = a[1] * b[2] - a[2] * b[1];
+  const double c1 = a[2] * b[0] - a[0] * b[2];
+  const double c2 = a[0] * b[1] - a[1] * b[0];
+  const double nc = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+  const double na = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+  return nc / (na * na * na);
+}
+
+// numpy.searchsorted(edges, v, side='right') with the histogramdd rule that
+// a value equal to the last edge falls into the last bin.  Returns the bin
+// or -1 for an outlier (including NaN).
+__device__ __forceinline__ int hist_bin(const double* __restrict__ e, int nb,
+                                        double v) {
+  int lo = 0, hi = nb + 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (e[mid] <= v)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  if (v == e[nb]) lo -= 1;
+  return (lo >= 1 && lo <= nb) ? lo - 1 : -1;
+}
+
+__global__ __launch_bounds__(256) void topo_epilogue_kernel(
+    const float4* __restrict__ ch, EpiArgs a) {
+  extern __shared__ unsigned int lh[];
+  const int nbins = a.nbd * a.nbc;
+  if (a.hist && a.lds_hist) {
+    for (int i = threadIdx.x; i < nbins; i += blockDim.x) lh[i] = 0u;
+    __syncthreads();
+  }
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l < a.L) {
+    double p[2][3], q1[2][3], q2[2][3];
+    for (int i = 0; i < 3; ++i) {
+      p[0][i] = (double)a.starts[3 * l + i];
+      p[1][i] = a.fin[3 * l + i];
+    }
+    step2_f64(p, q1, ch, a.npad, a.h);
+    step2_f64(q1, q2, ch, a.npad, a.h);
+    double kap[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      double v1[3], v2[3];
+      for (int i = 0; i < 3; ++i) {
+        v1[i] = q1[k][i] - p[k][i];
+        v2[i] = (q2[k][i] - 2.0 * q1[k][i]) + p[k][i];
+      }
+      kap[k] = curve_f64(v1, v2);
+    }
+    const double d0 = p[0][0] - p[1][0], d1 = p[0][1] - p[1][1],
+                 d2 = p[0][2] - p[1][2];
+    const double dist = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    const double curv = 0.5 * (kap[0] + kap[1]);
+    a.feats[2 * l] = dist;
+    a.feats[2 * l + 1] = curv;
+    if (a.hist) {
+      // bin the float32-rounded features, as the .top file stores them
+      const int bd = hist_bin(a.ed, a.nbd, (double)(float)dist);
+      const int bc = hist_bin(a.ec, a.nbc, (double)(float)curv);
+      if (bd >= 0 && bc >= 0) {
+        if (a.lds_hist)
+          atomicAdd(&lh[bd * a.nbc + bc], 1u);
+        else
+          atomicAdd(&a.hist[bd * a.nbc + bc], 1u);
+      }
+    }
+  }
+  if (a.hist && a.lds_hist) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < nbins; i += blockDim.x)
+      if (lh[i]) atomicAdd(&a.hist[i], lh[i]);
+  }
+}
+
+template <int S>
+hipError_t launch_prop(hipStream_t st, int G, const float4* ch, const PropArgs& a) {
+  hipLaunchKernelGGL((topo_propagate_kernel<S>), dim3(G), dim3(64 * S), 0, st, ch, a);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+int topo_run(cpet_ctx* ctx, int N, int L, const float* starts,
+             const int32_t* n_iter, float h, const float* dims, const float4* ch,
+             int npad, double* feats, int32_t* steps, int32_t* exited, int nb_d,
+             const double* edges_d, int nb_c, const double* edges_c,
+             uint32_t* hist, uint32_t flags) {
+  (void)N;
+  const bool dev = (flags & CPET_F_DEVICE) != 0;
+  hipStream_t st = ctx->stream;
+  int rc;
+  const void* dstarts = nullptr;
+  const void* diter = nullptr;
+  if ((rc = stage_in(ctx, kSlotPts, starts, (size_t)L * 3 * sizeof(float), dev, &dstarts))) return rc;
+  if ((rc = stage_in(ctx, kSlotIter, n_iter, (size_t)L * sizeof(int32_t), dev, &diter))) return rc;
+
+  // state: fin (L*3 double) | feats (L*2 double) | steps (L) | exited (L)
+  void* state = nullptr;
+  const size_t st_bytes = (size_t)L * (5 * sizeof(double) + 2 * sizeof(int32_t));
+  if ((rc = ctx_buf(ctx, kSlotTopoState, st_bytes, &state))) return rc;
+  double* d_fin = (double*)state;
+  double* d_feats = dev ? feats : d_fin + 3 * (size_t)L;
+  int32_t* d_steps = (int32_t*)(d_fin + 5 * (size_t)L);
+  int32_t* d_exited = d_steps + L;
+  if (dev && steps) d_steps = steps;
+  if (dev && exited) d_exited = exited;
+
+  // order (L ints) | bucket counts
+  void* ord = nullptr;
+  if ((rc = ctx_buf(ctx, kSlotTopoOrder, ((size_t)L + kNumBuckets) * sizeof(int32_t), &ord))) return rc;
+  int32_t* d_order = (int32_t*)ord;
+  unsigned int* d_buckets = (unsigned int*)(d_order + L);
+
+  // misc: counter | hist | edges
+  const int nbins = hist ? nb_d * nb_c : 0;
+  const size_t misc_bytes = 256 + (size_t)nbins * sizeof(uint32_t) +
+                            (hist ? (size_t)(nb_d + nb_c + 2) * sizeof(double) : 0) + 256;
+  void* misc = nullptr;
+  if ((rc = ctx_buf(ctx, kSlotTopoMisc, misc_bytes, &misc))) return rc;
+  unsigned int* d_counter = (unsigned int*)misc;
+  unsigned int* d_hist = nullptr;
+  const double* d_ed = nullptr;
+  const double* d_ec = nullptr;
+  if (hist) {
+    d_hist = dev ? (unsigned int*)hist : (unsigned int*)((char*)misc + 256);
+    double* e = (double*)((char*)misc + 256 + (((size_t)nbins * sizeof(uint32_t) + 255) & ~(size_t)255));
+    if (dev) {
+      d_ed = edges_d;
+      d_ec = edges_c;
+    } else {
+      CPET_HIP_TRY(hipMemcpyAsync(e, edges_d, (nb_d + 1) * sizeof(double), hipMemcpyHostToDevice, st));
+      CPET_HIP_TRY(hipMemcpyAsync(e + nb_d + 1, edges_c, (nb_c + 1) * sizeof(double), hipMemcpyHostToDevice, st));
+      d_ed = e;
+      d_ec = e + nb_d + 1;
+    }
+  }
+
+  // 1. longest-first order
+  if ((rc = prof_mark(ctx, 0))) return rc;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3((kNumBuckets + 255) / 256), dim3(256), 0, st,
+                     d_buckets, (long long)kNumBuckets, 0u);
+  hipLaunchKernelGGL(bucket_count_kernel, dim3((L + 255) / 256), dim3(256), 0, st, L,
+                     (const int32_t*)diter, d_buckets);
+  hipLaunchKernelGGL(bucket_scan_kernel, dim3(1), dim3(1024), 0, st, d_buckets);
+  hipLaunchKernelGGL(bucket_scatter_kernel, dim3((L + 255) / 256), dim3(256), 0, st, L,
+                     (const int32_t*)diter, d_buckets, d_order);
+  CPET_HIP_TRY(hipGetLastError());
+
+  // 2. propagate
+  int S = ctx->tune[2][0];
+  int G = ctx->tune[2][1];
+  if (S <= 0) S = 8;
+  const int gmax = (L + 63) / 64;
+  if (G <= 0) G = std::min(gmax, std::max(1, ctx->num_cus * 2));
+  G = std::min(G, gmax);
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(256), 0, st, d_counter, 1LL,
+                     (unsigned int)(G * 64));
+  if ((rc = prof_mark(ctx, 1))) return rc;
+  PropArgs pa;
+  pa.npad = npad;
+  pa.L = L;
+  pa.starts = (const float*)dstarts;
+  pa.n_iter = (const int32_t*)diter;
+  pa.order = d_order;
+  pa.h = (double)h;
+  pa.d0 = dims[0];
+  pa.d1 = dims[1];
+  pa.d2 = dims[2];
+  pa.fin = d_fin;
+  pa.steps = d_steps;
+  pa.exited = d_exited;
+  pa.counter = d_counter;
+  hipError_t e;
+  switch (S) {
+    case 1: e = launch_prop<1>(st, G, ch, pa); break;
+    case 2: e = launch_prop<2>(st, G, ch, pa); break;
+    case 4: e = launch_prop<4>(st, G, ch, pa); break;
+    case 8: e = launch_prop<8>(st, G, ch, pa); break;
+    case 16: e = launch_prop<16>(st, G, ch, pa); break;
+    default:
+      set_error("unsupported waves per workgroup %d (1,2,4,8,16)", S);
+      return CPET_ERR_ARG;
+  }
+  if (e != hipSuccess) {
+    set_error("topo propagate launch failed: %s", hipGetErrorString(e));
+    return CPET_ERR_HIP;
+  }
+
+  // 3. epilogue (+ histogram)
+  EpiArgs ea;
+  ea.L = L;
+  ea.npad = npad;
+  ea.starts = (const float*)dstarts;
+  ea.fin = d_fin;
+  ea.h = (double)h;
+  ea.feats = d_feats;
+  ea.ed = d_ed;
+  ea.nbd = nb_d;
+  ea.ec = d_ec;
+  ea.nbc = nb_c;
+  ea.hist = d_hist;
+  ea.lds_hist = (hist && nbins <= 8192) ? 1 : 0;
+  if ((rc = prof_mark(ctx, 2))) return rc;
+  if (hist)
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((nbins + 255) / 256), dim3(256), 0, st,
+                       d_hist, (long long)nbins, 0u);
+  const size_t lds = ea.lds_hist ? (size_t)nbins * sizeof(unsigned int) : 0;
+  hipLaunchKernelGGL(topo_epilogue_kernel, dim3((L + 255) / 256), dim3(256), lds, st, ch, ea);
+  CPET_HIP_TRY(hipGetLastError());
+  if ((rc = prof_mark(ctx, 3))) return rc;
+
+  if (!dev) {
+    CPET_HIP_TRY(hipMemcpyAsync(feats, d_feats, (size_t)L * 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (steps) CPET_HIP_TRY(hipMemcpyAsync(steps, d_steps, (size_t)L * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    if (exited) CPET_HIP_TRY(hipMemcpyAsync(exited, d_exited, (size_t)L * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    if (hist) CPET_HIP_TRY(hipMemcpyAsync(hist, d_hist, (size_t)nbins * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  }
+  if (!dev || !(flags & CPET_F_NO_SYNC)) {
+    CPET_HIP_TRY(hipStreamSynchronize(st));
+    if ((rc = prof_collect(ctx, 4))) return rc;
+  }
+  return CPET_OK;
+}
+
+}  // namespace cpet

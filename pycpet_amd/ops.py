@@ -1,0 +1,195 @@
+"""Array-level entry points over libcpet_hip.
+
+Each function accepts either numpy arrays (host; the library stages them to
+the device and copies results back) or torch tensors already on the GPU
+(passed as device pointers and run on torch's current stream, so inputs stay
+resident in HBM and results come back as device tensors).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Any, Optional
+
+import numpy as np
+
+from . import _native as nat
+
+
+def _is_torch(a: Any) -> bool:
+    return type(a).__module__.split(".")[0] == "torch"
+
+
+def _host_f32(a, shape_last=None) -> np.ndarray:
+    arr = np.ascontiguousarray(a, dtype=np.float32)
+    if shape_last is not None:
+        arr = arr.reshape(-1, shape_last)
+    return arr
+
+
+def _ptr(a) -> int:
+    if _is_torch(a):
+        return a.data_ptr()
+    return a.ctypes.data
+
+
+def _torch_dev_args(ctx: nat.Context, *tensors):
+    import torch
+
+    for t in tensors:
+        if t is None:
+            continue
+        if not t.is_cuda:
+            raise ValueError("torch inputs must be device tensors")
+        if not t.is_contiguous():
+            raise ValueError("torch inputs must be contiguous")
+    dev = tensors[0].device
+    ctx.set_stream(torch.cuda.current_stream(dev).cuda_stream)
+    return dev
+
+
+def _prec_flag(prec: str) -> int:
+    try:
+        return nat.PREC[prec]
+    except KeyError as exc:
+        raise ValueError(f"precision must be one of {sorted(nat.PREC)}") from exc
+
+
+def field_grid(ax, ay, az, x, q, *, want_E: bool = True, want_phi: bool = False,
+               soften: bool = True, prec: str = "tiled", out_f64: bool = False,
+               ctx: Optional[nat.Context] = None):
+    """Field and/or potential on the mesh ax x ay x az ('ij' order, z fastest).
+
+    Returns (E (M,3) or None, phi (M,) or None) with M = nx*ny*nz.
+    """
+    ctx = ctx or nat.default_context()
+    flags = _prec_flag(prec) | (nat.F_OUT_F64 if out_f64 else 0)
+    if _is_torch(x):
+        import torch
+
+        dev = _torch_dev_args(ctx, ax, ay, az, x, q)
+        nx, ny, nz = ax.numel(), ay.numel(), az.numel()
+        M = nx * ny * nz
+        odt = torch.float64 if out_f64 else torch.float32
+        E = torch.empty((M, 3), dtype=odt, device=dev) if want_E else None
+        phi = torch.empty((M,), dtype=odt, device=dev) if want_phi else None
+        flags |= nat.F_DEVICE | nat.F_NO_SYNC
+        N = q.numel()
+    else:
+        ax, ay, az = (_host_f32(v).ravel() for v in (ax, ay, az))
+        x = _host_f32(x, 3)
+        q = _host_f32(q).ravel()
+        nx, ny, nz = ax.size, ay.size, az.size
+        M = nx * ny * nz
+        odt = np.float64 if out_f64 else np.float32
+        E = np.empty((M, 3), dtype=odt) if want_E else None
+        phi = np.empty((M,), dtype=odt) if want_phi else None
+        N = q.size
+    if x.shape[0] != N:
+        raise ValueError(f"x has {x.shape[0]} rows but q has {N} charges")
+    nat.check(ctx.lib.cpet_field_grid(
+        ctx.handle, nx, ny, nz, _ptr(ax), _ptr(ay), _ptr(az), N, _ptr(x), _ptr(q),
+        _ptr(E) if E is not None else None, _ptr(phi) if phi is not None else None,
+        1 if soften else 0, flags), ctx.lib)
+    return E, phi
+
+
+def field_points(pts, x, q, *, want_E: bool = True, want_phi: bool = False,
+                 soften: bool = True, prec: str = "tiled", out_f64: bool = False,
+                 ctx: Optional[nat.Context] = None):
+    """Field and/or potential at arbitrary points pts (M,3)."""
+    ctx = ctx or nat.default_context()
+    flags = _prec_flag(prec) | (nat.F_OUT_F64 if out_f64 else 0)
+    if _is_torch(x):
+        import torch
+
+        dev = _torch_dev_args(ctx, pts, x, q)
+        M = pts.shape[0]
+        odt = torch.float64 if out_f64 else torch.float32
+        E = torch.empty((M, 3), dtype=odt, device=dev) if want_E else None
+        phi = torch.empty((M,), dtype=odt, device=dev) if want_phi else None
+        flags |= nat.F_DEVICE | nat.F_NO_SYNC
+        N = q.numel()
+    else:
+        pts = _host_f32(pts, 3)
+        x = _host_f32(x, 3)
+        q = _host_f32(q).ravel()
+        M = pts.shape[0]
+        odt = np.float64 if out_f64 else np.float32
+        E = np.empty((M, 3), dtype=odt) if want_E else None
+        phi = np.empty((M,), dtype=odt) if want_phi else None
+        N = q.size
+    if x.shape[0] != N:
+        raise ValueError(f"x has {x.shape[0]} rows but q has {N} charges")
+    nat.check(ctx.lib.cpet_field_points(
+        ctx.handle, M, N, _ptr(pts), _ptr(x), _ptr(q),
+        _ptr(E) if E is not None else None, _ptr(phi) if phi is not None else None,
+        1 if soften else 0, flags), ctx.lib)
+    return E, phi
+
+
+@dataclass
+class TopoResult:
+    feats: Any            # (L,2) float64: distance, mean curvature (input order)
+    steps: Any            # (L,) int32 Euler steps taken
+    exited: Any           # (L,) int32: 1 = left the box, 0 = hit n_iter
+    hist: Any = None      # (nb_d, nb_c) uint32 or None
+
+
+def topo(starts, n_iter, h: float, dims, x, q, *, edges=None,
+         ctx: Optional[nat.Context] = None) -> TopoResult:
+    """Streamlines from every start (thread_operation semantics).
+
+    edges: optional (edges_d, edges_c) increasing float64 bin edges; the
+    histogram counts the float32-rounded features with numpy.histogramdd
+    rules.
+    """
+    ctx = ctx or nat.default_context()
+    flags = 0
+    h = float(np.float32(h))
+    if _is_torch(x):
+        import torch
+
+        dev = _torch_dev_args(ctx, starts, n_iter, dims, x, q)
+        L = starts.shape[0]
+        feats = torch.empty((L, 2), dtype=torch.float64, device=dev)
+        steps = torch.empty((L,), dtype=torch.int32, device=dev)
+        exited = torch.empty((L,), dtype=torch.int32, device=dev)
+        hist = None
+        ed = ec = None
+        if edges is not None:
+            ed = torch.as_tensor(edges[0], dtype=torch.float64, device=dev).contiguous()
+            ec = torch.as_tensor(edges[1], dtype=torch.float64, device=dev).contiguous()
+            hist = torch.empty((ed.numel() - 1, ec.numel() - 1), dtype=torch.int32, device=dev)
+        flags |= nat.F_DEVICE | nat.F_NO_SYNC
+        N = q.numel()
+    else:
+        starts = _host_f32(starts, 3)
+        n_iter = np.ascontiguousarray(n_iter, dtype=np.int32).ravel()
+        dims = _host_f32(dims).ravel()
+        x = _host_f32(x, 3)
+        q = _host_f32(q).ravel()
+        L = starts.shape[0]
+        feats = np.empty((L, 2), dtype=np.float64)
+        steps = np.empty((L,), dtype=np.int32)
+        exited = np.empty((L,), dtype=np.int32)
+        hist = None
+        ed = ec = None
+        if edges is not None:
+            ed = np.ascontiguousarray(edges[0], dtype=np.float64).ravel()
+            ec = np.ascontiguousarray(edges[1], dtype=np.float64).ravel()
+            hist = np.empty((ed.size - 1, ec.size - 1), dtype=np.uint32)
+        N = q.size
+    if n_iter.shape[0] != L:
+        raise ValueError("n_iter must have one entry per start point")
+    if x.shape[0] != N:
+        raise ValueError(f"x has {x.shape[0]} rows but q has {N} charges")
+    nb_d = (ed.shape[0] - 1) if ed is not None else 0
+    nb_c = (ec.shape[0] - 1) if ec is not None else 0
+    nat.check(ctx.lib.cpet_topo(
+        ctx.handle, N, L, _ptr(starts), _ptr(n_iter), ctypes.c_float(h), _ptr(dims),
+        _ptr(x), _ptr(q), _ptr(feats), _ptr(steps), _ptr(exited),
+        nb_d, _ptr(ed) if ed is not None else None,
+        nb_c, _ptr(ec) if ec is not None else None,
+        _ptr(hist) if hist is not None else None, flags), ctx.lib)
+    return TopoResult(feats, steps, exited, hist)

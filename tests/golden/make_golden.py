@@ -1,0 +1,61 @@
+"""Writes the golden vectors in this directory from the reference's own
+native kernels (CPET/utils/math_module.c compiled under oracle/_ref by
+`make -C oracle ref`), driven the way the reference's ctypes wrappers drive
+them (CPET/utils/c_ops.py:250-341 and CPET/utils/calculator.py:430-475).
+
+Run in a checkout where the reference source exists:
+    make -C oracle ref && python tests/golden/make_golden.py
+
+Each .npz holds the seeded inputs and the reference's FP32 outputs only.
+"""
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+
+from oracle.oracle import Ref  # noqa: E402
+from pycpet_amd import workloads as W  # noqa: E402
+
+OUT = Path(__file__).resolve().parent
+
+
+def volume(ref):
+    x, q = W.ball_exclusion(500, 15.0, 1.0, 2.0, seed=11)
+    axes = W.grid_axes((1.0, 1.0, 1.0), 0.25)  # 9^3
+    g = np.stack(np.meshgrid(*axes, indexing="ij"), -1).reshape(-1, 3).astype(np.float32)
+    E = ref.compute_looped_field(g, x, q)
+    phi = ref.esp_points(g, x, q)
+    pf = ref.calc_field(np.zeros(3, np.float32), x, q)
+    np.savez_compressed(OUT / "ref_volume_500.npz", x=x, q=q, ax=axes[0], ay=axes[1],
+                        az=axes[2], E=E, phi=phi, point_field=pf)
+
+
+def volume_floor(ref):
+    # a charge 5e-4 A from a grid point exercises the r^2 >= 1e-6 floor
+    x, q = W.ball_exclusion(64, 6.0, 0.5, 1.0, seed=12)
+    axes = W.grid_axes((0.5, 0.5, 0.5), 0.25)  # 5^3
+    x = np.concatenate([x, np.array([[0.25, 0.0005, -0.25]], np.float32)])
+    q = np.concatenate([q, np.array([0.7], np.float32)])
+    g = np.stack(np.meshgrid(*axes, indexing="ij"), -1).reshape(-1, 3).astype(np.float32)
+    E = ref.compute_looped_field(g, x, q)
+    np.savez_compressed(OUT / "ref_volume_floor.npz", x=x, q=q, ax=axes[0], ay=axes[1],
+                        az=axes[2], E=E)
+
+
+def topo(ref):
+    c = W.small_topo(N=200, L=160, cap=300, seed=13, dims=(0.6, 0.6, 0.6), h=0.01)
+    ret = ref.topo(c.starts, c.n_iter, c.h, c.dims, c.x, c.q)
+    np.savez_compressed(OUT / "ref_topo_200.npz", x=c.x, q=c.q, starts=c.starts,
+                        n_iter=c.n_iter, h=np.float32(c.h), dims=c.dims, ret=ret)
+
+
+if __name__ == "__main__":
+    r = Ref()
+    volume(r)
+    volume_floor(r)
+    topo(r)
+    for p in sorted(OUT.glob("*.npz")):
+        print(p.name, p.stat().st_size)

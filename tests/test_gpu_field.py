@@ -1,0 +1,182 @@
+"""GPU parity of the field / potential kernels against the FP64 oracle.
+
+Tolerance (stated once): pointwise relative error
+    |E_gpu(p) - E_64(p)| / max(|E_64(p)|, 1e-3 * RMS_p |E_64|) <= 1e-5
+for E (vector norm) and phi, with E_64 the FP64 oracle on the same float32
+inputs.  The denominator floor keeps points where the field nearly cancels
+from turning FP32 rounding into unbounded relative error.
+"""
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def rel_err(a, b, floor_frac=1e-3):
+    a = np.asarray(a, float)
+    b = np.asarray(b, float)
+    if b.ndim == 1:
+        a, b = a[:, None], b[:, None]
+    nb = np.linalg.norm(b, axis=1)
+    d = np.linalg.norm(a - b, axis=1)
+    rms = np.sqrt(np.mean(nb ** 2))
+    return float((d / np.maximum(nb, floor_frac * rms)).max()), float(np.sqrt(np.mean(d ** 2)) / rms)
+
+
+def mesh(axes):
+    return np.stack(np.meshgrid(*axes, indexing="ij"), -1).reshape(-1, 3).astype(np.float32)
+
+
+def test_volume_2k_grid_vs_oracle(gpu_ctx, oracle_mod):
+    from pycpet_amd import ops, workloads as W
+
+    c = W.volume_2k()
+    print(f"case volume_2k N={c.N} M={c.M}", flush=True)
+    E, phi = ops.field_grid(*c.axes, c.x, c.q, want_E=True, want_phi=True, ctx=gpu_ctx)
+    E64, phi64 = oracle_mod.field_f64(mesh(c.axes), c.x, c.q, want_phi=True)
+    assert rel_err(E, E64)[0] <= TOL
+    assert rel_err(phi, phi64)[0] <= TOL
+
+
+def test_points_path_matches_grid_path(gpu_ctx):
+    from pycpet_amd import ops, workloads as W
+
+    c = W.volume_2k()
+    Eg, pg = ops.field_grid(*c.axes, c.x, c.q, want_phi=True, ctx=gpu_ctx)
+    Ep, pp = ops.field_points(mesh(c.axes), c.x, c.q, want_phi=True, ctx=gpu_ctx)
+    assert rel_err(Ep, Eg)[0] < 1e-6
+    assert rel_err(pp, pg)[0] < 1e-6
+
+
+def test_volume_30k_precision_modes(gpu_ctx, oracle_mod):
+    """Clustered 30k-charge case (net -3 e): every FP32 mode vs FP64 on a
+    seeded 3,000-point sample of the 101^3 grid, full grid computed."""
+    from pycpet_amd import ops, workloads as W
+
+    c = W.volume_30k()
+    print(f"case volume_30k N={c.N} M={c.M}", flush=True)
+    pts = mesh(c.axes)
+    idx = np.random.default_rng(0).choice(pts.shape[0], 3000, replace=False)
+    E64, phi64 = oracle_mod.field_f64(pts[idx], c.x, c.q, want_phi=True)
+    errs = {}
+    for prec in ("tiled", "kahan", "naive", "f64"):
+        E, phi = ops.field_grid(*c.axes, c.x, c.q, want_phi=True, prec=prec,
+                                out_f64=True, ctx=gpu_ctx)
+        errs[prec] = (rel_err(E[idx], E64), rel_err(phi[idx], phi64))
+        print(prec, errs[prec], flush=True)
+    for prec in ("tiled", "kahan"):
+        assert errs[prec][0][0] <= TOL and errs[prec][1][0] <= TOL, (prec, errs[prec])
+    assert errs["f64"][0][0] < 1e-11 and errs["f64"][1][0] < 1e-11
+    # float32 outputs of the default mode agree with its float64 outputs
+    E32, _ = ops.field_grid(*c.axes, c.x, c.q, ctx=gpu_ctx)
+    assert rel_err(E32[idx], E64)[0] <= TOL
+
+
+def test_reference_golden_vectors(gpu_ctx):
+    from pycpet_amd import ops
+
+    g = np.load(GOLDEN / "ref_volume_500.npz")
+    E, phi = ops.field_grid(g["ax"], g["ay"], g["az"], g["x"], g["q"], want_phi=True,
+                            ctx=gpu_ctx)
+    assert rel_err(E, g["E"])[0] < 5e-5  # reference is FP32 -ffast-math
+    assert rel_err(phi, g["phi"])[0] < 5e-5
+    f = np.load(GOLDEN / "ref_volume_floor.npz")
+    E, _ = ops.field_grid(f["ax"], f["ay"], f["az"], f["x"], f["q"], ctx=gpu_ctx)
+    assert rel_err(E, f["E"])[0] < 5e-5
+
+
+def test_dropin_entry_points(gpu_ctx, oracle_mod):
+    from pycpet_amd.c_ops import Math_ops
+
+    g = np.load(GOLDEN / "ref_volume_500.npz")
+    m = Math_ops()
+    pts = mesh([g["ax"], g["ay"], g["az"]])
+    E = m.compute_looped_field(pts, g["x"], g["q"].reshape(-1, 1))
+    assert E.dtype == np.float32 and E.shape == pts.shape
+    E64, phi64 = oracle_mod.field_f64(pts, g["x"], g["q"], want_phi=True)
+    assert rel_err(E, E64)[0] <= TOL
+    esp = m.calc_esp_base(pts[7], g["x"], g["q"])
+    assert esp.shape == (1,) and abs(esp[0] - phi64[7]) <= TOL * abs(phi64[7])
+    pf = m.calc_field(np.zeros(3, np.float32), g["x"], g["q"])
+    np.testing.assert_allclose(pf, g["point_field"], rtol=5e-5, atol=1e-6)
+    Eb = m.calc_field_base(np.zeros(3, np.float32), g["x"], g["q"])
+    np.testing.assert_allclose(Eb, pf, rtol=1e-6)
+
+
+def test_edge_cases(gpu_ctx, oracle_mod):
+    from pycpet_amd import ops
+    from pycpet_amd._native import CpetError
+
+    rng = np.random.default_rng(1)
+    # one charge, one point; ragged charge counts around the 64-charge tile
+    for N in (1, 63, 64, 65, 130):
+        x = rng.uniform(-4, 4, (N, 3)).astype(np.float32)
+        q = rng.normal(0, 0.5, N).astype(np.float32)
+        for axes in ([np.array([0.1], np.float32)] * 3,
+                     [np.linspace(-1, 1, 7, dtype=np.float32), np.linspace(-1, 1, 3, dtype=np.float32),
+                      np.linspace(-1, 1, 5, dtype=np.float32)]):
+            E, phi = ops.field_grid(*axes, x, q, want_phi=True, ctx=gpu_ctx)
+            E64, p64 = oracle_mod.field_f64(mesh(axes), x, q, want_phi=True)
+            assert rel_err(E, E64)[0] <= TOL and rel_err(phi, p64)[0] <= TOL, N
+    # a charge exactly on a grid point: phi is +-inf like 1/sqrt(0); E floored
+    x = np.array([[0.0, 0.0, 0.0], [2.0, 1.0, 0.5]], np.float32)
+    q = np.array([0.5, -0.3], np.float32)
+    axes = [np.array([-0.5, 0.0, 0.5], np.float32)] * 3
+    E, phi = ops.field_grid(*axes, x, q, want_phi=True, ctx=gpu_ctx)
+    c = 13  # the (0,0,0) grid point
+    assert np.isposinf(phi[c]) and np.all(np.isfinite(E))
+    # invalid sizes are rejected with an error, not a crash
+    with pytest.raises(CpetError):
+        ops.field_points(np.zeros((0, 3), np.float32), x, q, ctx=gpu_ctx)
+    with pytest.raises(CpetError):
+        ops.field_grid(np.zeros(0, np.float32), axes[1], axes[2], x, q, ctx=gpu_ctx)
+
+
+def test_charge_split_invariance(gpu_ctx):
+    """Splitting the charge sum over workgroups only regroups FP64 partials."""
+    from pycpet_amd import ops, workloads as W
+
+    c = W.volume_2k()
+    out = []
+    for splits in (1, 3, 16):
+        gpu_ctx.set_tuning(0, 4, splits)
+        E, phi = ops.field_grid(*c.axes, c.x, c.q, want_phi=True, out_f64=True, ctx=gpu_ctx)
+        out.append((E, phi))
+    gpu_ctx.set_tuning(0, 0, 0)
+    for E, phi in out[1:]:
+        np.testing.assert_allclose(E, out[0][0], rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(phi, out[0][1], rtol=1e-12, atol=1e-12)
+
+
+def test_points_per_lane_invariance(gpu_ctx):
+    from pycpet_amd import ops, workloads as W
+
+    c = W.volume_2k()
+    ref = None
+    for k in (1, 2, 3, 4, 6, 8):
+        gpu_ctx.set_tuning(0, k, 1)
+        E, _ = ops.field_grid(*c.axes, c.x, c.q, out_f64=True, ctx=gpu_ctx)
+        if ref is None:
+            ref = E
+        else:
+            np.testing.assert_array_equal(E, ref)  # same charge tiling: bitwise
+    gpu_ctx.set_tuning(0, 0, 0)
+
+
+def test_device_tensor_path(gpu_ctx):
+    import torch
+
+    from pycpet_amd import ops, workloads as W
+
+    c = W.volume_2k()
+    En, pn = ops.field_grid(*c.axes, c.x, c.q, want_phi=True, ctx=gpu_ctx)
+    dev = torch.device("cuda:0")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    Et, pt = ops.field_grid(*(t(a) for a in c.axes), t(c.x), t(c.q), want_phi=True, ctx=gpu_ctx)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(Et.cpu().numpy(), En)
+    np.testing.assert_array_equal(pt.cpu().numpy(), pn)

@@ -1,0 +1,180 @@
+"""GPU parity of the streamline kernels against the FP64 oracle.
+
+Matching rule (stated once): a streamline "matches" when its step count and
+stop kind (box exit / step limit) equal the oracle's — indexing and counting
+are bit-exact.  A one-ulp difference in a field sum can move a box crossing
+across a step boundary, so at least 99% of streamlines must match; on the
+matched set the features must agree within 1e-4 relative:
+    |d_gpu - d_64| <= 1e-4 |d_64|,  |k_gpu - k_64| <= 1e-4 max(|k_64|, 1e-6).
+Histogram counts must equal numpy.histogram2d of the GPU's own (float32-
+rounded) features exactly, and equal the oracle's histogram wherever the
+binned features match.
+"""
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+FEAT_TOL = 1e-4
+MATCH_MIN = 0.99
+
+
+def compare(r, feats64, steps64, exited64, label):
+    m = (r.steps == steps64) & (r.exited == exited64)
+    frac = float(m.mean())
+    dd = np.abs(r.feats[m, 0] - feats64[m, 0]) / np.maximum(np.abs(feats64[m, 0]), 1e-12)
+    dk = np.abs(r.feats[m, 1] - feats64[m, 1]) / np.maximum(np.abs(feats64[m, 1]), 1e-6)
+    print(f"{label}: match {frac:.5f} dist max {dd.max():.3e} curv max {dk.max():.3e}", flush=True)
+    assert frac >= MATCH_MIN, frac
+    assert dd.max() <= FEAT_TOL and dk.max() <= FEAT_TOL
+    return m
+
+
+def test_small_vs_oracle(gpu_ctx, oracle_mod):
+    from pycpet_amd import ops, workloads as W
+
+    c = W.small_topo(N=300, L=512, cap=400, seed=5)
+    print(f"case {c.name}", flush=True)
+    r = ops.topo(c.starts, c.n_iter, c.h, c.dims, c.x, c.q, ctx=gpu_ctx)
+    f64, s64, e64 = oracle_mod.topo_f64(c.starts, c.n_iter, c.h, c.dims, c.x, c.q)
+    compare(r, f64, s64, e64, c.name)
+
+
+def test_h1e3_vs_oracle(gpu_ctx, oracle_mod):
+    """h = 1e-3 A as in the 100k-streamline workload, on a case the CPU oracle
+    finishes in seconds."""
+    from pycpet_amd import ops, workloads as W
+
+    c = W.small_topo(N=800, L=384, cap=1500, seed=6, dims=(1.5, 1.5, 1.5), h=0.001, R=30.0)
+    print(f"case {c.name} h=1e-3", flush=True)
+    r = ops.topo(c.starts, c.n_iter, c.h, c.dims, c.x, c.q, ctx=gpu_ctx)
+    f64, s64, e64 = oracle_mod.topo_f64(c.starts, c.n_iter, c.h, c.dims, c.x, c.q)
+    compare(r, f64, s64, e64, c.name)
+
+
+def test_radial_lines_exact_steps(gpu_ctx):
+    from pycpet_amd import ops
+    from test_oracle_topo import _radial_expect
+
+    rng = np.random.default_rng(7)
+    dims = np.array([0.6, 0.5, 0.4], np.float32)
+    h = 0.01
+    c = np.array([3.0, 0.5, -0.4])
+    for sign in (+1.0, -1.0):
+        starts = rng.uniform(-0.35, 0.35, (300, 3)).astype(np.float32)
+        n_iter = rng.integers(1, 200, 300).astype(np.int32)
+        r = ops.topo(starts, n_iter, h, dims, c[None].astype(np.float32),
+                     np.array([sign * 0.8], np.float32), ctx=gpu_ctx)
+        exp_steps, clean, exp_exit = _radial_expect(
+            starts, c.astype(np.float32).astype(float), sign, dims.astype(float),
+            float(np.float32(h)), n_iter)
+        np.testing.assert_array_equal(r.steps[clean], exp_steps[clean])
+        np.testing.assert_array_equal(r.exited[clean].astype(bool), exp_exit[clean])
+        np.testing.assert_allclose(r.feats[:, 0], r.steps * float(np.float32(h)), rtol=1e-6)
+        assert np.abs(r.feats[:, 1]).max() < 1e-3
+
+
+def test_histogram_exact(gpu_ctx, oracle_mod):
+    from pycpet_amd import ops, workloads as W
+
+    c = W.small_topo(N=300, L=1024, cap=400, seed=8)
+    f64, s64, e64 = oracle_mod.topo_f64(c.starts, c.n_iter, c.h, c.dims, c.x, c.q)
+    ed = np.linspace(f64[:, 0].min(), f64[:, 0].max(), 21)
+    ec = np.linspace(f64[:, 1].min(), f64[:, 1].max(), 21)
+    r = ops.topo(c.starts, c.n_iter, c.h, c.dims, c.x, c.q, edges=(ed, ec), ctx=gpu_ctx)
+    fr = r.feats.astype(np.float32).astype(np.float64)
+    Hn, _, _ = np.histogram2d(fr[:, 0], fr[:, 1], bins=[ed, ec])
+    np.testing.assert_array_equal(r.hist, Hn.astype(np.uint32))
+    # vs the oracle: identical counts over streamlines whose binned features agree
+    fo = f64.astype(np.float32).astype(np.float64)
+    same_bin = (np.searchsorted(ed, fr[:, 0], "right") == np.searchsorted(ed, fo[:, 0], "right")) & \
+               (np.searchsorted(ec, fr[:, 1], "right") == np.searchsorted(ec, fo[:, 1], "right"))
+    assert same_bin.mean() >= MATCH_MIN
+    Ha, _, _ = np.histogram2d(fr[same_bin, 0], fr[same_bin, 1], bins=[ed, ec])
+    Hb = oracle_mod.histogram2d(fo[same_bin, 0], fo[same_bin, 1], ed, ec)
+    np.testing.assert_array_equal(Ha.astype(np.int64), Hb)
+
+
+def test_schedule_invariance(gpu_ctx):
+    """Which slot or workgroup runs a streamline does not change its result;
+    the charge-slice count only regroups FP64 partial sums."""
+    from pycpet_amd import ops, workloads as W
+
+    c = W.small_topo(N=700, L=900, cap=300, seed=9)
+    runs = {}
+    for S, G in ((8, 1), (8, 3), (8, 0), (1, 0), (4, 2), (16, 0)):
+        gpu_ctx.set_tuning(2, S, G)
+        runs[(S, G)] = ops.topo(c.starts, c.n_iter, c.h, c.dims, c.x, c.q, ctx=gpu_ctx)
+    gpu_ctx.set_tuning(2, 0, 0)
+    base = runs[(8, 0)]
+    for key in ((8, 1), (8, 3)):
+        np.testing.assert_array_equal(runs[key].feats, base.feats)
+        np.testing.assert_array_equal(runs[key].steps, base.steps)
+    for key in ((1, 0), (4, 2), (16, 0)):
+        m = runs[key].steps == base.steps
+        assert m.mean() >= MATCH_MIN
+        np.testing.assert_allclose(runs[key].feats[m], base.feats[m], rtol=1e-6, atol=1e-9)
+
+
+def test_reference_golden_vectors(gpu_ctx):
+    from pycpet_amd import ops
+
+    g = np.load(GOLDEN / "ref_topo_200.npz")
+    r = ops.topo(g["starts"], g["n_iter"], float(g["h"]), g["dims"], g["x"], g["q"], ctx=gpu_ctx)
+    ref = g["ret"].astype(float)
+    rd = np.abs(ref[:, 0] - r.feats[:, 0]) / np.maximum(r.feats[:, 0], 1e-12)
+    assert np.mean(rd < 1e-4) >= 0.95
+    ok = rd < 1e-4
+    assert np.median(np.abs(ref[ok, 1] - r.feats[ok, 1]) / np.abs(r.feats[ok, 1])) < 1e-2
+
+
+def test_edge_cases(gpu_ctx, oracle_mod):
+    from pycpet_amd import ops
+    from pycpet_amd._native import CpetError
+    from pycpet_amd.c_ops import Math_ops
+
+    x = np.array([[2.0, 1.0, -1.5], [-1.7, 0.4, 2.2]], np.float32)
+    q = np.array([0.3, -0.6], np.float32)
+    dims = np.full(3, 1.0, np.float32)
+    starts = np.array([[0.1, 0.0, 0.2], [0.0, 0.1, 0.0], [0.99, 0.0, 0.0], [0.2, 0.3, -0.1]],
+                      np.float32)
+    n_iter = np.array([1, 0, 5, 2000], np.int32)
+    r = ops.topo(starts, n_iter, 0.01, dims, x, q, ctx=gpu_ctx)
+    f64, s64, e64 = oracle_mod.topo_f64(starts, n_iter, 0.01, dims, x, q)
+    np.testing.assert_array_equal(r.steps, s64)
+    np.testing.assert_array_equal(r.exited, e64)
+    np.testing.assert_allclose(r.feats, f64, rtol=1e-5, atol=1e-9)
+    assert r.steps[1] == 0 and r.feats[1, 0] == 0.0
+    # single streamline through the reference-named entry point
+    ret = Math_ops().thread_operation(starts[3], 2000, x, q, 0.01, dims)
+    assert ret.dtype == np.float32 and ret.shape == (2,)
+    np.testing.assert_allclose(ret, f64[3].astype(np.float32), rtol=1e-5)
+    with pytest.raises(CpetError):
+        ops.topo(starts, n_iter, float("nan"), dims, x, q, ctx=gpu_ctx)
+    with pytest.raises(CpetError):
+        ops.topo(starts, n_iter, 0.01, np.array([1.0, -1.0, 1.0], np.float32), x, q, ctx=gpu_ctx)
+
+
+def test_full_size_100k(gpu_ctx, oracle_mod):
+    """The 5k-charge / 100k-streamline workload at full size: invariants over
+    all streamlines plus a seeded 48-streamline sample against the oracle."""
+    from pycpet_amd import ops, workloads as W
+
+    c = W.topo_5k()
+    print(f"case {c.name} N={c.N} L={c.L}", flush=True)
+    r = ops.topo(c.starts, c.n_iter, c.h, c.dims, c.x, c.q, ctx=gpu_ctx)
+    assert np.all(r.steps >= 1) and np.all(r.steps <= c.n_iter)
+    assert np.all(r.steps[r.exited == 0] == c.n_iter[r.exited == 0])
+    h = float(np.float32(c.h))
+    assert np.all(r.feats[:, 0] <= r.steps * h * (1 + 1e-9))  # chord <= arc
+    assert np.all(np.isfinite(r.feats))
+    idx = np.random.default_rng(0).choice(c.L, 48, replace=False)
+    f64, s64, e64 = oracle_mod.topo_f64(c.starts[idx], c.n_iter[idx], c.h, c.dims, c.x, c.q)
+    sub = ops.TopoResult(r.feats[idx], r.steps[idx], r.exited[idx])
+    m = (sub.steps == s64) & (sub.exited == e64)
+    assert m.mean() >= 0.95  # 48 samples: allow one boundary tie
+    dd = np.abs(sub.feats[m, 0] - f64[m, 0]) / f64[m, 0]
+    dk = np.abs(sub.feats[m, 1] - f64[m, 1]) / np.maximum(np.abs(f64[m, 1]), 1e-6)
+    assert dd.max() <= FEAT_TOL and dk.max() <= FEAT_TOL
