@@ -41,12 +41,15 @@ extern "C" {
 #define CPET_F_DEVICE 0x1u   /* array arguments are device pointers */
 #define CPET_F_OUT_F64 0x2u  /* field/phi outputs are double, not float */
 #define CPET_F_NO_SYNC 0x4u  /* device-pointer calls: do not wait for the stream */
-/* precision of the FP32 field kernels, bits 8..9 */
-#define CPET_F_PREC_TILED (0u << 8) /* FP32 per 64-charge tile, FP64 across (default) */
+/* precision of the field kernels, bits 8..10 */
+#define CPET_F_PREC_AUTO (0u << 8)  /* E: FP32 tiled; phi: mixed (default) */
 #define CPET_F_PREC_NAIVE (1u << 8) /* one FP32 running sum */
-#define CPET_F_PREC_KAHAN (2u << 8) /* Kahan FP32 within a tile, FP64 across */
+#define CPET_F_PREC_KAHAN (2u << 8) /* Kahan FP32 within a 64-charge tile, FP64 across */
 #define CPET_F_PREC_F64 (3u << 8)   /* FP64 throughout */
-#define CPET_F_PREC_MASK (3u << 8)
+#define CPET_F_PREC_TILED (4u << 8) /* FP32 per 64-charge tile, FP64 across */
+#define CPET_F_PREC_MIXED (5u << 8) /* FP64 displacement and r^2, FP32 rsqrt seed +
+                                       one FP64 Newton step, FP64 sums */
+#define CPET_F_PREC_MASK (7u << 8)
 
 /* ============ Part 1: reference-compatible entry points (host arrays) ==== */
 

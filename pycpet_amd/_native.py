@@ -26,7 +26,8 @@ CPET_ERR_BUDGET = -5
 F_DEVICE = 0x1
 F_OUT_F64 = 0x2
 F_NO_SYNC = 0x4
-PREC = {"tiled": 0 << 8, "naive": 1 << 8, "kahan": 2 << 8, "f64": 3 << 8}
+PREC = {"auto": 0 << 8, "naive": 1 << 8, "kahan": 2 << 8, "f64": 3 << 8,
+        "tiled": 4 << 8, "mixed": 5 << 8}
 
 
 class CpetError(RuntimeError):
@@ -75,12 +76,36 @@ SIGNATURES = {
 }
 
 
+def _share_torch_hip_runtime() -> None:
+    """Makes torch's bundled HIP runtime the process's HIP runtime.
+
+    torch's libraries ask for `libamdhip64.so` from torch/lib, while this
+    library asks for the `libamdhip64.so.7` soname.  If this library were
+    loaded first, importing torch later would map a second HIP runtime into
+    the process (its device enumeration then fails).  Loading torch's copy
+    first with RTLD_GLOBAL makes both resolve to that one runtime, so device
+    pointers from torch tensors are valid here.  Without torch installed the
+    system ROCm runtime is used.
+    """
+    import importlib.util
+
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.submodule_search_locations:
+        return
+    for loc in spec.submodule_search_locations:
+        rt = Path(loc) / "lib" / "libamdhip64.so"
+        if rt.exists():
+            ctypes.CDLL(str(rt), mode=ctypes.RTLD_GLOBAL)
+            return
+
+
 def load(path: str | os.PathLike | None = None):
     """Loads (once) and returns the library with argtypes declared."""
     global _lib
     with _lock:
         if _lib is not None and path is None:
             return _lib
+        _share_torch_hip_runtime()
         p = Path(path) if path is not None else LIB_PATH
         if not p.exists():
             raise CpetNotBuilt(

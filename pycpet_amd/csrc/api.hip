@@ -15,9 +15,9 @@ namespace cpet {
 // Implemented in field.hip / topo.hip.
 int field_launch(cpet_ctx* ctx, bool grid_mode, int nx, int ny, int nz,
                  const float* ax, const float* ay, const float* az, long long M,
-                 const float* pts, const float4* ch, int npad, void* E_out,
-                 void* phi_out, int soften, uint32_t flags, int forced_k,
-                 int forced_splits);
+                 const float* pts, const float4* ch, const double4* chd, int npad,
+                 void* E_out, void* phi_out, int soften, uint32_t flags,
+                 int forced_k, int forced_splits);
 int topo_run(cpet_ctx* ctx, int N, int L, const float* starts,
              const int32_t* n_iter, float h, const float* dims, const float4* ch,
              int npad, double* feats, int32_t* steps, int32_t* exited, int nb_d,
@@ -31,13 +31,16 @@ cpet_ctx* g_default = nullptr;
 
 __global__ __launch_bounds__(256) void pack_charges_kernel(
     int N, int npad, const float* __restrict__ x, const float* __restrict__ q,
-    float4* __restrict__ out) {
+    float4* __restrict__ out, double4* __restrict__ out64) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= npad) return;
+  float4 c;
   if (j < N)
-    out[j] = make_float4(x[3 * j], x[3 * j + 1], x[3 * j + 2], q[j]);
+    c = make_float4(x[3 * j], x[3 * j + 1], x[3 * j + 2], q[j]);
   else
-    out[j] = make_float4(kPadCoord, kPadCoord, kPadCoord, 0.f);
+    c = make_float4(kPadCoord, kPadCoord, kPadCoord, 0.f);
+  out[j] = c;
+  if (out64) out64[j] = make_double4(c.x, c.y, c.z, c.w);
 }
 
 bool finite3(const float* v) {
@@ -103,7 +106,7 @@ int stage_in(cpet_ctx* ctx, int slot, const void* src, size_t bytes, bool dev,
 }
 
 int pack_charges(cpet_ctx* ctx, int N, const float* x, const float* q,
-                 bool dev, const float4** out, int* npad) {
+                 bool dev, const float4** out, const double4** out64, int* npad) {
   const int np = round_up(N > 0 ? N : 1, kTile);
   const void* dx = nullptr;
   const void* dq = nullptr;
@@ -114,11 +117,17 @@ int pack_charges(cpet_ctx* ctx, int N, const float* x, const float* q,
   void* d = nullptr;
   rc = ctx_buf(ctx, kSlotCharges, (size_t)np * sizeof(float4), &d);
   if (rc) return rc;
+  void* d64 = nullptr;
+  if (out64) {
+    rc = ctx_buf(ctx, kSlotCharges64, (size_t)np * sizeof(double4), &d64);
+    if (rc) return rc;
+  }
   hipLaunchKernelGGL(pack_charges_kernel, dim3((np + 255) / 256), dim3(256), 0,
                      ctx->stream, N, np, (const float*)dx, (const float*)dq,
-                     (float4*)d);
+                     (float4*)d, (double4*)d64);
   CPET_HIP_TRY(hipGetLastError());
   *out = (const float4*)d;
+  if (out64) *out64 = (const double4*)d64;
   *npad = np;
   return CPET_OK;
 }
@@ -329,8 +338,9 @@ static int field_common(cpet_ctx* ctx, bool grid_mode, int nx, int ny, int nz,
   int rc = ctx_activate(ctx);
   if (rc) return rc;
   const float4* ch = nullptr;
+  const double4* chd = nullptr;
   int npad = 0;
-  rc = pack_charges(ctx, N, x, q, dev, &ch, &npad);
+  rc = pack_charges(ctx, N, x, q, dev, &ch, &chd, &npad);
   if (rc) return rc;
   const void *dax = nullptr, *day = nullptr, *daz = nullptr, *dpts = nullptr;
   if (grid_mode) {
@@ -369,7 +379,7 @@ static int field_common(cpet_ctx* ctx, bool grid_mode, int nx, int ny, int nz,
   const int kind = grid_mode ? 0 : 1;
   rc = field_launch(ctx, grid_mode, nx, ny, nz, (const float*)dax,
                     (const float*)day, (const float*)daz, M,
-                    (const float*)dpts, ch, npad, E_out ? dE : nullptr,
+                    (const float*)dpts, ch, chd, npad, E_out ? dE : nullptr,
                     phi_out ? dphi : nullptr, soften, flags, ctx->tune[kind][0],
                     ctx->tune[kind][1]);
   if (rc) return rc;
@@ -451,7 +461,7 @@ int cpet_topo(cpet_ctx* ctx, int N, int L, const float* starts,
   if (rc) return rc;
   const float4* ch = nullptr;
   int npad = 0;
-  rc = pack_charges(ctx, N, x, q, dev, &ch, &npad);
+  rc = pack_charges(ctx, N, x, q, dev, &ch, nullptr, &npad);
   if (rc) return rc;
   return topo_run(ctx, N, L, starts, n_iter, h, hdims, ch, npad, feats, steps,
                   exited, nb_d, edges_d, nb_c, edges_c, hist, flags);

@@ -23,12 +23,14 @@ constexpr double kCoulomb = 14.3996451;
 constexpr int kTile = 64;
 constexpr float kPadCoord = 1.0e18f;
 
-// Precision modes of the FP32 field kernels (bits 8..9 of the flags word).
-enum Prec : int {
-  kPrecTiled = 0,  // FP32 per 64-charge tile, FP64 across tiles (default)
-  kPrecNaive = 1,  // one FP32 running sum over all charges
-  kPrecKahan = 2,  // Kahan-compensated FP32 inside a tile, FP64 across tiles
-  kPrecF64 = 3,    // everything in FP64 (diagnostics)
+// Precision modes of the field kernels (bits 8..10 of the flags word).
+enum Mode : int {
+  kModeAuto = 0,   // E: FP32 tiled; phi: mixed (default)
+  kModeNaive = 1,  // one FP32 running sum over all charges
+  kModeKahan = 2,  // Kahan-compensated FP32 inside a tile, FP64 across tiles
+  kModeF64 = 3,    // everything in FP64
+  kModeTiled = 4,  // FP32 per 64-charge tile, FP64 across tiles
+  kModeMixed = 5,  // FP64 d and r^2, FP32 rsq seed + FP64 Newton, FP64 sums
 };
 
 // What a field launch computes.
@@ -84,6 +86,7 @@ namespace cpet {
 
 enum Slot : int {
   kSlotCharges = 0,  // packed float4 charges
+  kSlotCharges64,    // the same charges as double4
   kSlotPts,          // staged points / starts
   kSlotAxes,         // grid axes
   kSlotOut0,         // staged outputs
@@ -112,10 +115,11 @@ int stage_in(cpet_ctx* ctx, int slot, const void* src, size_t bytes, bool dev,
 // Global default context for the reference-compatible entry points.
 int default_ctx(cpet_ctx** out);
 
-// Packs x (N,3) and q (N) into padded float4 charges; returns the padded
-// count through npad.  x/q may be host or device pointers (dev flag).
+// Packs x (N,3) and q (N) into padded float4 charges (and, when out64 is
+// non-null, a double4 copy); returns the padded count through npad.  x/q may
+// be host or device pointers (dev flag).
 int pack_charges(cpet_ctx* ctx, int N, const float* x, const float* q,
-                 bool dev, const float4** out, int* npad);
+                 bool dev, const float4** out, const double4** out64, int* npad);
 
 inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
 

@@ -5,19 +5,23 @@
 // calc_field (:255-293, E, no floor).  The reference runs one FP32 point at
 // a time on one CPU thread; here one GPU lane owns K points, every lane of a
 // wave walks the same charge, and charges are read through the scalar unit
-// (wave-uniform addresses -> s_load), so the vector ALUs only do pair math:
+// (wave-uniform addresses -> s_load), so the vector ALUs only do pair math.
 //
-//   grid kernel (points on a tensor-product mesh, K points along z per lane)
-//     per charge: dx, dy, dxy2 = dy*dy + dx*dx                (4 ops / K)
-//     per pair:   dz, r2 = dz*dz + dxy2, rinv = rsq(r2),
-//                 [min(rinv, rsq(1e-6))], s = (q*re)*(re*re),
-//                 3 FMA into E, [1 FMA into phi]               (~9-11 ops)
-//   points kernel (arbitrary points): 3 subtracts + 3 ops for r2 instead.
+// Grid kernel (tensor-product mesh): a lane owns K consecutive z-points of
+// one (x, y) line, so dx, dy and dx^2 + dy^2 are computed once per charge:
+//   per pair (FP32 E):  dz, r2 = dz*dz + dxy2, rinv = rsq(r2),
+//                       min(rinv, rsq(1e-6)), s = (q*re)*(re*re), 3 FMA
+// Points kernel (arbitrary points): 3 subtracts + 3 ops for r2 instead.
 //
-// Accumulation: FP32 inside a 64-charge tile, FP64 across tiles (default),
-// or naive FP32 / Kahan FP32 / FP64 for the precision study.  Small grids
-// split the charges over blockIdx.y and reduce the FP64 partials in a fixed
-// order in a second kernel.
+// Precision is chosen per quantity (flags bits 8..10):
+//   auto   E: FP32 pair terms, FP32 sum per 64-charge tile, FP64 across tiles
+//          phi: "mixed" — FP64 displacement and r^2 (exact for FP32 inputs),
+//          an FP32 rsq seed refined by one FP64 Newton step, FP64 sums.
+//          (phi cancels strongly between charges of both signs, so FP32
+//          pair rounding alone exceeds a 1e-5 pointwise bound near phi ~ 0.)
+//   tiled / kahan / naive / f64 / mixed: the same scheme for both.
+// Small grids split the charges over blockIdx.y and reduce the FP64 partials
+// in a fixed order in a second kernel.
 
 #include <cmath>
 #include <cstdio>
@@ -26,6 +30,25 @@
 
 namespace cpet {
 namespace {
+
+// per-quantity precision
+enum QPrec : int { kQNaive = 0, kQKahan = 1, kQTiled = 2, kQF64 = 3, kQMixed = 4 };
+
+__host__ __device__ constexpr int qprec_e(int mode) {
+  return mode == kModeNaive ? kQNaive
+       : mode == kModeKahan ? kQKahan
+       : mode == kModeF64   ? kQF64
+       : mode == kModeMixed ? kQMixed
+                            : kQTiled;  // auto, tiled
+}
+__host__ __device__ constexpr int qprec_p(int mode) {
+  return mode == kModeNaive ? kQNaive
+       : mode == kModeKahan ? kQKahan
+       : mode == kModeF64   ? kQF64
+       : mode == kModeTiled ? kQTiled
+                            : kQMixed;  // auto, mixed
+}
+__host__ __device__ constexpr bool q_is32(int q) { return q <= kQTiled; }
 
 __device__ __forceinline__ double rsqrt_f64(double x) {
   // Hardware estimate refined by two Newton-Raphson steps for 1/sqrt(x):
@@ -37,26 +60,15 @@ __device__ __forceinline__ double rsqrt_f64(double x) {
   return (x == 0.0) ? y : y1;
 }
 
-// One FP32 pair term.  WHAT selects field (floored or not) and/or phi.
-template <int WHAT>
-__device__ __forceinline__ void pair_f32(float dx, float dy, float dz,
-                                         float dxy2, float q, float rclamp,
-                                         float& ex, float& ey, float& ez,
-                                         float& ph) {
-  (void)dy;
-  const float r2 = fmaf(dz, dz, dxy2);
-  const float ri = __builtin_amdgcn_rsqf(r2);
-  if (WHAT == kWhatPhi || WHAT == kWhatESoftPhi) ph = fmaf(q, ri, ph);
-  if (WHAT != kWhatPhi) {
-    const float re = (WHAT == kWhatEHard) ? ri : fminf(ri, rclamp);
-    const float s = (q * re) * (re * re);
-    ex = fmaf(s, dx, ex);
-    ey = fmaf(s, dy, ey);
-    ez = fmaf(s, dz, ez);
-  }
+// One Newton-Raphson step in FP64 from an FP32 estimate of 1/sqrt(x):
+// relative error ~1e-7 -> ~1e-14.  Clamping y*y keeps x == 0 (y = +inf)
+// from producing 0*inf: the residual is then 1 and the result stays +inf.
+__device__ __forceinline__ double rsqrt_refine(float seed, double x) {
+  const double y = (double)seed;
+  const double e = fma(-x, fmin(y * y, 1.7976931348623157e308), 1.0);
+  return fma(0.5 * y, e, y);
 }
 
-// Kahan-compensated accumulation of one term.
 __device__ __forceinline__ void kahan_add(float t, float& s, float& c) {
   const float y = t - c;
   const float u = s + y;
@@ -64,208 +76,140 @@ __device__ __forceinline__ void kahan_add(float t, float& s, float& c) {
   s = u;
 }
 
-template <int WHAT>
-__device__ __forceinline__ void pair_kahan(float dx, float dy, float dz,
-                                           float dxy2, float q, float rclamp,
-                                           float (&acc)[4], float (&cmp)[4]) {
-  const float r2 = fmaf(dz, dz, dxy2);
-  const float ri = __builtin_amdgcn_rsqf(r2);
-  if (WHAT == kWhatPhi || WHAT == kWhatESoftPhi) kahan_add(q * ri, acc[3], cmp[3]);
-  if (WHAT != kWhatPhi) {
-    const float re = (WHAT == kWhatEHard) ? ri : fminf(ri, rclamp);
-    const float s = (q * re) * (re * re);
-    kahan_add(s * dx, acc[0], cmp[0]);
-    kahan_add(s * dy, acc[1], cmp[1]);
-    kahan_add(s * dz, acc[2], cmp[2]);
-  }
-}
-
-template <int WHAT>
-__device__ __forceinline__ void pair_f64(double px, double py, double pz,
-                                         float4 c, double (&acc)[4]) {
-  const double dx = px - (double)c.x;
-  const double dy = py - (double)c.y;
-  const double dz = pz - (double)c.z;
-  const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-  const double q = (double)c.w;
-  if (WHAT == kWhatPhi || WHAT == kWhatESoftPhi) acc[3] += q * rsqrt_f64(r2);
-  if (WHAT != kWhatPhi) {
-    const double floor2 = (double)1e-6f;
-    const double r2e = (WHAT == kWhatEHard) ? r2 : fmax(r2, floor2);
-    const double re = rsqrt_f64(r2e);
-    const double s = (q * re) * (re * re);
-    acc[0] = fma(s, dx, acc[0]);
-    acc[1] = fma(s, dy, acc[1]);
-    acc[2] = fma(s, dz, acc[2]);
-  }
-}
-
-// Accumulates the field of charges [t0*kTile, t1*kTile) at K points that
-// share (px, py) and differ in z.  For the arbitrary-point kernel K points
-// are passed with their own x/y through the generic path below.
-template <int K, int PREC, int WHAT>
-__device__ __forceinline__ void accumulate_zline(
-    float px, float py, const float (&pz)[K], const float4* __restrict__ ch,
-    int t0, int t1, float rclamp, double (&out)[K][4]) {
-#pragma unroll
-  for (int k = 0; k < K; ++k)
-#pragma unroll
-    for (int c = 0; c < 4; ++c) out[k][c] = 0.0;
-
-  if constexpr (PREC == kPrecF64) {
-    for (int j = t0 * kTile; j < t1 * kTile; ++j) {
-      const float4 c = ch[j];
-#pragma unroll
-      for (int k = 0; k < K; ++k) pair_f64<WHAT>(px, py, pz[k], c, out[k]);
-    }
-  } else if constexpr (PREC == kPrecNaive) {
-    float a[K][4];
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-#pragma unroll
-      for (int c = 0; c < 4; ++c) a[k][c] = 0.f;
-    for (int j = t0 * kTile; j < t1 * kTile; ++j) {
-      const float4 c = ch[j];
-      const float dx = px - c.x, dy = py - c.y;
-      const float dxy2 = fmaf(dy, dy, dx * dx);
-#pragma unroll
-      for (int k = 0; k < K; ++k)
-        pair_f32<WHAT>(dx, dy, pz[k] - c.z, dxy2, c.w, rclamp, a[k][0],
-                       a[k][1], a[k][2], a[k][3]);
-    }
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-#pragma unroll
-      for (int c = 0; c < 4; ++c) out[k][c] = (double)a[k][c];
-  } else if constexpr (PREC == kPrecKahan) {
-    for (int t = t0; t < t1; ++t) {
-      float a[K][4], cm[K][4];
-#pragma unroll
-      for (int k = 0; k < K; ++k)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) a[k][c] = cm[k][c] = 0.f;
-      const float4* cp = ch + t * kTile;
-#pragma unroll 4
-      for (int j = 0; j < kTile; ++j) {
-        const float4 c = cp[j];
-        const float dx = px - c.x, dy = py - c.y;
-        const float dxy2 = fmaf(dy, dy, dx * dx);
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-          pair_kahan<WHAT>(dx, dy, pz[k] - c.z, dxy2, c.w, rclamp, a[k], cm[k]);
-      }
-#pragma unroll
-      for (int k = 0; k < K; ++k)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) out[k][c] += (double)a[k][c] - (double)cm[k][c];
-    }
-  } else {  // kPrecTiled
-    for (int t = t0; t < t1; ++t) {
-      float a[K][4];
-#pragma unroll
-      for (int k = 0; k < K; ++k)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) a[k][c] = 0.f;
-      const float4* cp = ch + t * kTile;
-#pragma unroll 16
-      for (int j = 0; j < kTile; ++j) {
-        const float4 c = cp[j];
-        const float dx = px - c.x, dy = py - c.y;
-        const float dxy2 = fmaf(dy, dy, dx * dx);
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-          pair_f32<WHAT>(dx, dy, pz[k] - c.z, dxy2, c.w, rclamp, a[k][0],
-                         a[k][1], a[k][2], a[k][3]);
-      }
-#pragma unroll
-      for (int k = 0; k < K; ++k)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) out[k][c] += (double)a[k][c];
-    }
-  }
-}
-
-// Per-point generic accumulation (each of the K points has its own x, y).
-template <int K, int PREC, int WHAT>
-__device__ __forceinline__ void accumulate_points(
+// Accumulates the unscaled sums (sum q d / r^3 and sum q / r) of charges
+// [t0*kTile, t1*kTile) at K points into out[k][0..3] (FP64).
+// SHARED_XY: all K points share px[0], py[0] (grid z-line).
+template <int K, bool SHARED_XY, int MODE, int WHAT>
+__device__ __forceinline__ void accumulate(
     const float (&px)[K], const float (&py)[K], const float (&pz)[K],
-    const float4* __restrict__ ch, int t0, int t1, float rclamp,
-    double (&out)[K][4]) {
+    const float4* __restrict__ ch, const double4* __restrict__ chd, int t0,
+    int t1, double (&out)[K][4]) {
+  constexpr bool hasE = WHAT != kWhatPhi;
+  constexpr bool hasP = WHAT == kWhatPhi || WHAT == kWhatESoftPhi;
+  constexpr bool soft = WHAT == kWhatESoft || WHAT == kWhatESoftPhi;
+  constexpr int QE = qprec_e(MODE), QP = qprec_p(MODE);
+  constexpr bool e32 = hasE && q_is32(QE), p32 = hasP && q_is32(QP);
+  constexpr bool e64 = hasE && !q_is32(QE), p64 = hasP && !q_is32(QP);
+  constexpr bool need32 = e32 || p32;
+  constexpr bool need64 = e64 || p64;
+  constexpr bool full64 = MODE == kModeF64;
+  constexpr bool naive = MODE == kModeNaive;
+  constexpr bool kahan = MODE == kModeKahan;
+  constexpr int kUnroll = (need64 || kahan) ? 4 : 16;
+
+  const float rclamp = __builtin_amdgcn_rsqf(1e-6f);
+  const double rclampd = rsqrt_f64((double)1e-6f);
+  double pxd[K], pyd[K], pzd[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    pxd[k] = (double)px[SHARED_XY ? 0 : k];
+    pyd[k] = (double)py[SHARED_XY ? 0 : k];
+    pzd[k] = (double)pz[k];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) out[k][c] = 0.0;
+  }
+  float na[K][4];  // naive: FP32 sums over the whole charge range
 #pragma unroll
   for (int k = 0; k < K; ++k)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) out[k][c] = 0.0;
-  if constexpr (PREC == kPrecF64) {
-    for (int j = t0 * kTile; j < t1 * kTile; ++j) {
-      const float4 c = ch[j];
-#pragma unroll
-      for (int k = 0; k < K; ++k) pair_f64<WHAT>(px[k], py[k], pz[k], c, out[k]);
-    }
-  } else if constexpr (PREC == kPrecNaive) {
-    float a[K][4];
+    for (int c = 0; c < 4; ++c) na[k][c] = 0.f;
+
+  for (int t = t0; t < t1; ++t) {
+    float ta[K][4], tc[K][4];  // per-tile FP32 sums (+ Kahan compensation)
 #pragma unroll
     for (int k = 0; k < K; ++k)
 #pragma unroll
-      for (int c = 0; c < 4; ++c) a[k][c] = 0.f;
-    for (int j = t0 * kTile; j < t1 * kTile; ++j) {
+      for (int c = 0; c < 4; ++c) ta[k][c] = tc[k][c] = 0.f;
+    const int jb = t * kTile;
+#pragma unroll kUnroll
+    for (int jj = 0; jj < kTile; ++jj) {
+      const int j = jb + jj;
       const float4 c = ch[j];
+      float sdx = 0.f, sdy = 0.f, sdxy2 = 0.f;
+      double sdxd = 0.0, sdyd = 0.0, sdxy2d = 0.0;
+      double4 cd = make_double4(0.0, 0.0, 0.0, 0.0);
+      if constexpr (need64) cd = chd[j];
+      if constexpr (SHARED_XY && need32) {
+        sdx = px[0] - c.x;
+        sdy = py[0] - c.y;
+        sdxy2 = fmaf(sdy, sdy, sdx * sdx);
+      }
+      if constexpr (SHARED_XY && need64) {
+        sdxd = pxd[0] - cd.x;
+        sdyd = pyd[0] - cd.y;
+        sdxy2d = fma(sdyd, sdyd, sdxd * sdxd);
+      }
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        const float dx = px[k] - c.x, dy = py[k] - c.y;
-        pair_f32<WHAT>(dx, dy, pz[k] - c.z, fmaf(dy, dy, dx * dx), c.w, rclamp,
-                       a[k][0], a[k][1], a[k][2], a[k][3]);
+        float ri = 0.f;
+        if constexpr (need32) {
+          const float dx = SHARED_XY ? sdx : px[k] - c.x;
+          const float dy = SHARED_XY ? sdy : py[k] - c.y;
+          const float dxy2 = SHARED_XY ? sdxy2 : fmaf(dy, dy, dx * dx);
+          const float dz = pz[k] - c.z;
+          const float r2 = fmaf(dz, dz, dxy2);
+          ri = __builtin_amdgcn_rsqf(r2);
+          float* acc = naive ? na[k] : ta[k];
+          if constexpr (p32) {
+            if constexpr (kahan)
+              kahan_add(c.w * ri, acc[3], tc[k][3]);
+            else
+              acc[3] = fmaf(c.w, ri, acc[3]);
+          }
+          if constexpr (e32) {
+            const float re = soft ? fminf(ri, rclamp) : ri;
+            const float s = (c.w * re) * (re * re);
+            if constexpr (kahan) {
+              kahan_add(s * dx, acc[0], tc[k][0]);
+              kahan_add(s * dy, acc[1], tc[k][1]);
+              kahan_add(s * dz, acc[2], tc[k][2]);
+            } else {
+              acc[0] = fmaf(s, dx, acc[0]);
+              acc[1] = fmaf(s, dy, acc[1]);
+              acc[2] = fmaf(s, dz, acc[2]);
+            }
+          }
+        }
+        if constexpr (need64) {
+          const double dx = SHARED_XY ? sdxd : pxd[k] - cd.x;
+          const double dy = SHARED_XY ? sdyd : pyd[k] - cd.y;
+          const double dxy2 = SHARED_XY ? sdxy2d : fma(dy, dy, dx * dx);
+          const double dz = pzd[k] - cd.z;
+          const double r2 = fma(dz, dz, dxy2);
+          double y;
+          if constexpr (full64)
+            y = rsqrt_f64(r2);
+          else
+            y = rsqrt_refine(need32 ? ri : __builtin_amdgcn_rsqf((float)r2), r2);
+          if constexpr (p64) out[k][3] = fma(cd.w, y, out[k][3]);
+          if constexpr (e64) {
+            const double re = soft ? fmin(y, rclampd) : y;
+            const double s = (cd.w * re) * (re * re);
+            out[k][0] = fma(s, dx, out[k][0]);
+            out[k][1] = fma(s, dy, out[k][1]);
+            out[k][2] = fma(s, dz, out[k][2]);
+          }
+        }
       }
     }
+    if constexpr (!naive && need32) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        if constexpr (e32) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+            out[k][c] += kahan ? (double)ta[k][c] - (double)tc[k][c] : (double)ta[k][c];
+        }
+        if constexpr (p32)
+          out[k][3] += kahan ? (double)ta[k][3] - (double)tc[k][3] : (double)ta[k][3];
+      }
+    }
+  }
+  if constexpr (naive) {
 #pragma unroll
     for (int k = 0; k < K; ++k)
 #pragma unroll
-      for (int c = 0; c < 4; ++c) out[k][c] = (double)a[k][c];
-  } else if constexpr (PREC == kPrecKahan) {
-    for (int t = t0; t < t1; ++t) {
-      float a[K][4], cm[K][4];
-#pragma unroll
-      for (int k = 0; k < K; ++k)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) a[k][c] = cm[k][c] = 0.f;
-      const float4* cp = ch + t * kTile;
-#pragma unroll 4
-      for (int j = 0; j < kTile; ++j) {
-        const float4 c = cp[j];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-          const float dx = px[k] - c.x, dy = py[k] - c.y;
-          pair_kahan<WHAT>(dx, dy, pz[k] - c.z, fmaf(dy, dy, dx * dx), c.w,
-                           rclamp, a[k], cm[k]);
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < K; ++k)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) out[k][c] += (double)a[k][c] - (double)cm[k][c];
-    }
-  } else {
-    for (int t = t0; t < t1; ++t) {
-      float a[K][4];
-#pragma unroll
-      for (int k = 0; k < K; ++k)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) a[k][c] = 0.f;
-      const float4* cp = ch + t * kTile;
-#pragma unroll 16
-      for (int j = 0; j < kTile; ++j) {
-        const float4 c = cp[j];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-          const float dx = px[k] - c.x, dy = py[k] - c.y;
-          pair_f32<WHAT>(dx, dy, pz[k] - c.z, fmaf(dy, dy, dx * dx), c.w,
-                         rclamp, a[k][0], a[k][1], a[k][2], a[k][3]);
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < K; ++k)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) out[k][c] += (double)a[k][c];
-    }
+      for (int c = 0; c < 4; ++c) out[k][c] += (double)na[k][c];
   }
 }
 
@@ -311,12 +255,12 @@ __device__ __forceinline__ void store_point(const FieldOut& o, long long m,
 }
 
 // Grid kernel: lane -> (line (i,j), chunk of K z-points).
-template <int K, int PREC, int WHAT>
+template <int K, int MODE, int WHAT>
 __global__ __launch_bounds__(256) void field_grid_kernel(
     int nx, int ny, int nz, int nzc, const float* __restrict__ ax,
     const float* __restrict__ ay, const float* __restrict__ az,
-    const float4* __restrict__ ch, int tiles_per_split, int ntiles,
-    FieldOut o) {
+    const float4* __restrict__ ch, const double4* __restrict__ chd,
+    int tiles_per_split, int ntiles, FieldOut o) {
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long lanes = (long long)nx * ny * nzc;
   if (gid >= lanes) return;
@@ -325,16 +269,18 @@ __global__ __launch_bounds__(256) void field_grid_kernel(
   const int i = (int)(line / ny);
   const int j = (int)(line - (long long)i * ny);
   const int z0 = chunk * K;
-  const float px = ax[i], py = ay[j];
-  float pz[K];
+  float px[K], py[K], pz[K];
 #pragma unroll
-  for (int k = 0; k < K; ++k) pz[k] = az[min(z0 + k, nz - 1)];
+  for (int k = 0; k < K; ++k) {
+    px[k] = ax[i];
+    py[k] = ay[j];
+    pz[k] = az[min(z0 + k, nz - 1)];
+  }
   const int split = blockIdx.y;
   const int t0 = split * tiles_per_split;
   const int t1 = min(t0 + tiles_per_split, ntiles);
-  const float rclamp = __builtin_amdgcn_rsqf(1e-6f);
   double acc[K][4];
-  accumulate_zline<K, PREC, WHAT>(px, py, pz, ch, t0, t1, rclamp, acc);
+  accumulate<K, true, MODE, WHAT>(px, py, pz, ch, chd, t0, t1, acc);
   const long long base = line * nz + z0;
 #pragma unroll
   for (int k = 0; k < K; ++k)
@@ -342,10 +288,11 @@ __global__ __launch_bounds__(256) void field_grid_kernel(
 }
 
 // Points kernel: lane owns points base + k*blockDim + tid.
-template <int K, int PREC, int WHAT>
+template <int K, int MODE, int WHAT>
 __global__ __launch_bounds__(256) void field_points_kernel(
     long long M, const float* __restrict__ pts, const float4* __restrict__ ch,
-    int tiles_per_split, int ntiles, FieldOut o) {
+    const double4* __restrict__ chd, int tiles_per_split, int ntiles,
+    FieldOut o) {
   const long long base = (long long)blockIdx.x * blockDim.x * K + threadIdx.x;
   float px[K], py[K], pz[K];
 #pragma unroll
@@ -359,9 +306,8 @@ __global__ __launch_bounds__(256) void field_points_kernel(
   const int split = blockIdx.y;
   const int t0 = split * tiles_per_split;
   const int t1 = min(t0 + tiles_per_split, ntiles);
-  const float rclamp = __builtin_amdgcn_rsqf(1e-6f);
   double acc[K][4];
-  accumulate_points<K, PREC, WHAT>(px, py, pz, ch, t0, t1, rclamp, acc);
+  accumulate<K, false, MODE, WHAT>(px, py, pz, ch, chd, t0, t1, acc);
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const long long m = base + (long long)k * blockDim.x;
@@ -389,80 +335,61 @@ __global__ __launch_bounds__(256) void field_reduce_kernel(long long M,
   store_point<WHAT>(d, m, acc, 0);
 }
 
-template <int WHAT>
-void launch_reduce(hipStream_t st, long long M, int splits, const FieldOut& o) {
-  const int blocks = (int)((M + 255) / 256);
-  hipLaunchKernelGGL((field_reduce_kernel<WHAT>), dim3(blocks), dim3(256), 0,
-                     st, M, splits, o);
+// ---- dispatch ---------------------------------------------------------
+
+struct Launch {
+  hipStream_t st;
+  dim3 grid;
+  bool grid_mode;
+  int nx, ny, nz, nzc;
+  const float *ax, *ay, *az, *pts;
+  long long M;
+  const float4* ch;
+  const double4* chd;
+  int tps, ntiles;
+  FieldOut o;
+};
+
+template <int K, int MODE, int WHAT>
+void launch_t(const Launch& L) {
+  if (L.grid_mode)
+    hipLaunchKernelGGL((field_grid_kernel<K, MODE, WHAT>), L.grid, dim3(256), 0, L.st,
+                       L.nx, L.ny, L.nz, L.nzc, L.ax, L.ay, L.az, L.ch, L.chd, L.tps,
+                       L.ntiles, L.o);
+  else
+    hipLaunchKernelGGL((field_points_kernel<K, MODE, WHAT>), L.grid, dim3(256), 0, L.st,
+                       L.M, L.pts, L.ch, L.chd, L.tps, L.ntiles, L.o);
 }
 
-// ---- dispatch tables --------------------------------------------------
-
-template <int K, int PREC, int WHAT>
-void launch_grid_t(hipStream_t st, dim3 grid, int nx, int ny, int nz, int nzc,
-                   const float* ax, const float* ay, const float* az,
-                   const float4* ch, int tps, int ntiles, const FieldOut& o) {
-  hipLaunchKernelGGL((field_grid_kernel<K, PREC, WHAT>), grid, dim3(256), 0,
-                     st, nx, ny, nz, nzc, ax, ay, az, ch, tps, ntiles, o);
-}
-
-template <int K, int PREC, int WHAT>
-void launch_points_t(hipStream_t st, dim3 grid, long long M, const float* pts,
-                     const float4* ch, int tps, int ntiles, const FieldOut& o) {
-  hipLaunchKernelGGL((field_points_kernel<K, PREC, WHAT>), grid, dim3(256), 0,
-                     st, M, pts, ch, tps, ntiles, o);
-}
-
-template <int PREC, int WHAT>
-bool launch_grid_k(int K, hipStream_t st, dim3 grid, int nx, int ny, int nz,
-                   int nzc, const float* ax, const float* ay, const float* az,
-                   const float4* ch, int tps, int ntiles, const FieldOut& o) {
-  switch (K) {
-    case 1: launch_grid_t<1, PREC, WHAT>(st, grid, nx, ny, nz, nzc, ax, ay, az, ch, tps, ntiles, o); return true;
-    case 2: launch_grid_t<2, PREC, WHAT>(st, grid, nx, ny, nz, nzc, ax, ay, az, ch, tps, ntiles, o); return true;
-    case 3: launch_grid_t<3, PREC, WHAT>(st, grid, nx, ny, nz, nzc, ax, ay, az, ch, tps, ntiles, o); return true;
-    case 4: launch_grid_t<4, PREC, WHAT>(st, grid, nx, ny, nz, nzc, ax, ay, az, ch, tps, ntiles, o); return true;
-    case 6: launch_grid_t<6, PREC, WHAT>(st, grid, nx, ny, nz, nzc, ax, ay, az, ch, tps, ntiles, o); return true;
-    case 8: launch_grid_t<8, PREC, WHAT>(st, grid, nx, ny, nz, nzc, ax, ay, az, ch, tps, ntiles, o); return true;
-    default: return false;
+// K choices: every listed K for the grid kernel in the default mode; 1, 2, 4
+// otherwise (the arbitrary-points kernel and the study modes).
+template <int MODE, int WHAT>
+bool launch_k(int K, const Launch& L) {
+  if (MODE == kModeAuto && L.grid_mode) {
+    switch (K) {
+      case 3: launch_t<3, MODE, WHAT>(L); return true;
+      case 6: launch_t<6, MODE, WHAT>(L); return true;
+      case 8: launch_t<8, MODE, WHAT>(L); return true;
+      default: break;
+    }
   }
-}
-
-template <int PREC, int WHAT>
-bool launch_points_k(int K, hipStream_t st, dim3 grid, long long M,
-                     const float* pts, const float4* ch, int tps, int ntiles,
-                     const FieldOut& o) {
   switch (K) {
-    case 1: launch_points_t<1, PREC, WHAT>(st, grid, M, pts, ch, tps, ntiles, o); return true;
-    case 2: launch_points_t<2, PREC, WHAT>(st, grid, M, pts, ch, tps, ntiles, o); return true;
-    case 4: launch_points_t<4, PREC, WHAT>(st, grid, M, pts, ch, tps, ntiles, o); return true;
+    case 1: launch_t<1, MODE, WHAT>(L); return true;
+    case 2: launch_t<2, MODE, WHAT>(L); return true;
+    case 4: launch_t<4, MODE, WHAT>(L); return true;
     default: return false;
   }
 }
 
 template <int WHAT>
-bool launch_grid_w(int prec, int K, hipStream_t st, dim3 grid, int nx, int ny,
-                   int nz, int nzc, const float* ax, const float* ay,
-                   const float* az, const float4* ch, int tps, int ntiles,
-                   const FieldOut& o) {
-  switch (prec) {
-    case kPrecTiled: return launch_grid_k<kPrecTiled, WHAT>(K, st, grid, nx, ny, nz, nzc, ax, ay, az, ch, tps, ntiles, o);
-    case kPrecNaive: return launch_grid_k<kPrecNaive, WHAT>(K, st, grid, nx, ny, nz, nzc, ax, ay, az, ch, tps, ntiles, o);
-    case kPrecKahan: return launch_grid_k<kPrecKahan, WHAT>(K, st, grid, nx, ny, nz, nzc, ax, ay, az, ch, tps, ntiles, o);
-    case kPrecF64:   return launch_grid_k<kPrecF64, WHAT>(K, st, grid, nx, ny, nz, nzc, ax, ay, az, ch, tps, ntiles, o);
-  }
-  return false;
-}
-
-template <int WHAT>
-bool launch_points_w(int prec, int K, hipStream_t st, dim3 grid, long long M,
-                     const float* pts, const float4* ch, int tps, int ntiles,
-                     const FieldOut& o) {
-  switch (prec) {
-    case kPrecTiled: return launch_points_k<kPrecTiled, WHAT>(K, st, grid, M, pts, ch, tps, ntiles, o);
-    case kPrecNaive: return launch_points_k<kPrecNaive, WHAT>(K, st, grid, M, pts, ch, tps, ntiles, o);
-    case kPrecKahan: return launch_points_k<kPrecKahan, WHAT>(K, st, grid, M, pts, ch, tps, ntiles, o);
-    case kPrecF64:   return launch_points_k<kPrecF64, WHAT>(K, st, grid, M, pts, ch, tps, ntiles, o);
+bool launch_m(int mode, int K, const Launch& L) {
+  switch (mode) {
+    case kModeAuto: return launch_k<kModeAuto, WHAT>(K, L);
+    case kModeNaive: return launch_k<kModeNaive, WHAT>(K, L);
+    case kModeKahan: return launch_k<kModeKahan, WHAT>(K, L);
+    case kModeF64: return launch_k<kModeF64, WHAT>(K, L);
+    case kModeTiled: return launch_k<kModeTiled, WHAT>(K, L);
+    case kModeMixed: return launch_k<kModeMixed, WHAT>(K, L);
   }
   return false;
 }
@@ -487,18 +414,24 @@ int pick_splits(long long lanes, int ntiles, int num_cus, int forced) {
 
 }  // namespace
 
+bool field_mode_valid(int mode) { return mode >= kModeAuto && mode <= kModeMixed; }
+
 // Shared implementation of the grid and points entry points.
 int field_launch(cpet_ctx* ctx, bool grid_mode, int nx, int ny, int nz,
                  const float* ax, const float* ay, const float* az, long long M,
-                 const float* pts, const float4* ch, int npad, void* E_out,
-                 void* phi_out, int soften, uint32_t flags, int forced_k,
-                 int forced_splits) {
+                 const float* pts, const float4* ch, const double4* chd, int npad,
+                 void* E_out, void* phi_out, int soften, uint32_t flags,
+                 int forced_k, int forced_splits) {
   const int what = what_of(E_out != nullptr, phi_out != nullptr, soften);
   if (what < 0) {
     set_error("E and phi together require soften=1 (volume semantics)");
     return CPET_ERR_ARG;
   }
-  const int prec = (int)((flags & CPET_F_PREC_MASK) >> 8);
+  const int mode = (int)((flags & CPET_F_PREC_MASK) >> 8);
+  if (!field_mode_valid(mode)) {
+    set_error("unknown precision mode %d", mode);
+    return CPET_ERR_ARG;
+  }
   const int ntiles = npad / kTile;
   FieldOut o;
   o.e = E_out;
@@ -510,22 +443,23 @@ int field_launch(cpet_ctx* ctx, bool grid_mode, int nx, int ny, int nz,
   int K;
   long long lanes;
   int nzc = 0;
+  const bool heavy = mode != kModeAuto && mode != kModeTiled && mode != kModeNaive;
   if (grid_mode) {
     if (forced_k > 0) {
       K = forced_k;
-    } else if (prec == kPrecF64 || prec == kPrecKahan) {
+    } else if (heavy) {
       K = 2;
     } else {
       // choose K so the z-line splits evenly and enough waves remain
       K = 4;
       const long long l4 = (long long)nx * ny * ((nz + 3) / 4);
       if (l4 < (long long)ctx->num_cus * 64 * 4) K = 2;
-      if (nz % 3 == 0 && nz % 4 != 0 && K == 4) K = 3;
+      if (mode == kModeAuto && nz % 3 == 0 && nz % 4 != 0 && K == 4) K = 3;
     }
     nzc = (nz + K - 1) / K;
     lanes = (long long)nx * ny * nzc;
   } else {
-    K = forced_k > 0 ? forced_k : (prec == kPrecF64 || prec == kPrecKahan ? 1 : 2);
+    K = forced_k > 0 ? forced_k : (heavy ? 1 : 2);
     lanes = (M + K - 1) / K;
   }
   const int splits = pick_splits(lanes, ntiles, ctx->num_cus, forced_splits);
@@ -539,41 +473,46 @@ int field_launch(cpet_ctx* ctx, bool grid_mode, int nx, int ny, int nz,
   }
   const long long blocks_x = grid_mode ? (lanes + 255) / 256
                                        : (M + 256LL * K - 1) / (256LL * K);
-  dim3 g((unsigned)blocks_x, (unsigned)splits);
-  bool ok = false;
-  hipStream_t st = ctx->stream;
+  Launch L;
+  L.st = ctx->stream;
+  L.grid = dim3((unsigned)blocks_x, (unsigned)splits);
+  L.grid_mode = grid_mode;
+  L.nx = nx;
+  L.ny = ny;
+  L.nz = nz;
+  L.nzc = nzc;
+  L.ax = ax;
+  L.ay = ay;
+  L.az = az;
+  L.pts = pts;
+  L.M = M;
+  L.ch = ch;
+  L.chd = chd;
+  L.tps = tps;
+  L.ntiles = ntiles;
+  L.o = o;
   int prc = prof_mark(ctx, 0);
   if (prc) return prc;
+  bool ok = false;
   switch (what) {
-    case kWhatESoft:
-      ok = grid_mode ? launch_grid_w<kWhatESoft>(prec, K, st, g, nx, ny, nz, nzc, ax, ay, az, ch, tps, ntiles, o)
-                     : launch_points_w<kWhatESoft>(prec, K, st, g, M, pts, ch, tps, ntiles, o);
-      break;
-    case kWhatPhi:
-      ok = grid_mode ? launch_grid_w<kWhatPhi>(prec, K, st, g, nx, ny, nz, nzc, ax, ay, az, ch, tps, ntiles, o)
-                     : launch_points_w<kWhatPhi>(prec, K, st, g, M, pts, ch, tps, ntiles, o);
-      break;
-    case kWhatESoftPhi:
-      ok = grid_mode ? launch_grid_w<kWhatESoftPhi>(prec, K, st, g, nx, ny, nz, nzc, ax, ay, az, ch, tps, ntiles, o)
-                     : launch_points_w<kWhatESoftPhi>(prec, K, st, g, M, pts, ch, tps, ntiles, o);
-      break;
-    case kWhatEHard:
-      ok = grid_mode ? launch_grid_w<kWhatEHard>(prec, K, st, g, nx, ny, nz, nzc, ax, ay, az, ch, tps, ntiles, o)
-                     : launch_points_w<kWhatEHard>(prec, K, st, g, M, pts, ch, tps, ntiles, o);
-      break;
+    case kWhatESoft: ok = launch_m<kWhatESoft>(mode, K, L); break;
+    case kWhatPhi: ok = launch_m<kWhatPhi>(mode, K, L); break;
+    case kWhatESoftPhi: ok = launch_m<kWhatESoftPhi>(mode, K, L); break;
+    case kWhatEHard: ok = launch_m<kWhatEHard>(mode, K, L); break;
   }
   if (!ok) {
-    set_error("unsupported points-per-lane %d", K);
+    set_error("unsupported points-per-lane %d for precision mode %d", K, mode);
     return CPET_ERR_ARG;
   }
   CPET_HIP_TRY(hipGetLastError());
   if ((prc = prof_mark(ctx, 1))) return prc;
   if (splits > 1) {
+    const int blocks = (int)((M + 255) / 256);
     switch (what) {
-      case kWhatESoft: launch_reduce<kWhatESoft>(st, M, splits, o); break;
-      case kWhatPhi: launch_reduce<kWhatPhi>(st, M, splits, o); break;
-      case kWhatESoftPhi: launch_reduce<kWhatESoftPhi>(st, M, splits, o); break;
-      case kWhatEHard: launch_reduce<kWhatEHard>(st, M, splits, o); break;
+      case kWhatESoft: hipLaunchKernelGGL((field_reduce_kernel<kWhatESoft>), dim3(blocks), dim3(256), 0, L.st, M, splits, o); break;
+      case kWhatPhi: hipLaunchKernelGGL((field_reduce_kernel<kWhatPhi>), dim3(blocks), dim3(256), 0, L.st, M, splits, o); break;
+      case kWhatESoftPhi: hipLaunchKernelGGL((field_reduce_kernel<kWhatESoftPhi>), dim3(blocks), dim3(256), 0, L.st, M, splits, o); break;
+      case kWhatEHard: hipLaunchKernelGGL((field_reduce_kernel<kWhatEHard>), dim3(blocks), dim3(256), 0, L.st, M, splits, o); break;
     }
     CPET_HIP_TRY(hipGetLastError());
   }

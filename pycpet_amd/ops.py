@@ -56,7 +56,7 @@ def _prec_flag(prec: str) -> int:
 
 
 def field_grid(ax, ay, az, x, q, *, want_E: bool = True, want_phi: bool = False,
-               soften: bool = True, prec: str = "tiled", out_f64: bool = False,
+               soften: bool = True, prec: str = "auto", out_f64: bool = False,
                ctx: Optional[nat.Context] = None):
     """Field and/or potential on the mesh ax x ay x az ('ij' order, z fastest).
 
@@ -95,7 +95,7 @@ def field_grid(ax, ay, az, x, q, *, want_E: bool = True, want_phi: bool = False,
 
 
 def field_points(pts, x, q, *, want_E: bool = True, want_phi: bool = False,
-                 soften: bool = True, prec: str = "tiled", out_f64: bool = False,
+                 soften: bool = True, prec: str = "auto", out_f64: bool = False,
                  ctx: Optional[nat.Context] = None):
     """Field and/or potential at arbitrary points pts (M,3)."""
     ctx = ctx or nat.default_context()

@@ -1,0 +1,57 @@
+"""Summarises a rocprofv3 kernel-trace database (``*_results.db``) as text.
+
+    python -m pycpet_amd.profsummary <results.db> [--pmc]
+
+Prints one row per kernel: calls, mean / min / max duration and total time,
+and with --pmc the mean of each collected counter per dispatch.  The profile
+summaries kept with the repository are produced with this.
+"""
+from __future__ import annotations
+
+import argparse
+import sqlite3
+import sys
+
+
+def kernel_rows(db: str):
+    con = sqlite3.connect(db)
+    cur = con.cursor()
+    rows = cur.execute(
+        "select name, count(*), avg(end-start), min(end-start), max(end-start), "
+        "sum(end-start) from kernels group by name order by sum(end-start) desc").fetchall()
+    con.close()
+    return rows
+
+
+def pmc_rows(db: str):
+    con = sqlite3.connect(db)
+    cur = con.cursor()
+    try:
+        rows = cur.execute(
+            "select kernel_name, counter_name, count(*), avg(value) from pmc_events "
+            "group by kernel_name, counter_name order by kernel_name").fetchall()
+    except sqlite3.Error:
+        cols = [r[1] for r in cur.execute("pragma table_info(pmc_events)")]
+        raise SystemExit(f"unexpected pmc_events layout: {cols}")
+    con.close()
+    return rows
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("db")
+    ap.add_argument("--pmc", action="store_true")
+    a = ap.parse_args(argv)
+    print(f"{'kernel':72s} {'calls':>5s} {'mean_ms':>10s} {'min_ms':>10s} {'max_ms':>10s} {'total_ms':>10s}")
+    for name, n, avg, mn, mx, tot in kernel_rows(a.db):
+        print(f"{name[:72]:72s} {n:5d} {avg / 1e6:10.3f} {mn / 1e6:10.3f} {mx / 1e6:10.3f} {tot / 1e6:10.2f}")
+    if a.pmc:
+        print()
+        print(f"{'kernel':72s} {'counter':24s} {'dispatches':>10s} {'mean':>16s}")
+        for kname, cname, n, avg in pmc_rows(a.db):
+            print(f"{kname[:72]:72s} {cname:24s} {n:10d} {avg:16.1f}")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
