@@ -149,6 +149,47 @@ int cpet_topo(cpet_ctx *ctx, int N, int L, const float *starts,
               int32_t *exited, int nb_d, const double *edges_d, int nb_c,
               const double *edges_c, uint32_t *hist, uint32_t flags);
 
+/* ---- frame preparation (filters, box-frame transform, compaction) ----
+ * dtype codes: 0 = float32, 1 = float64.  `x` holds N rows of 3 values in
+ * x_dtype; arithmetic on (x - center) runs in sub_dtype (numpy's promotion of
+ * the two operands); center / Tinv / box_shift / dims are passed as doubles
+ * holding values of that type.  Outputs are caller-allocated; host or device
+ * pointers as for the other calls (CPET_F_DEVICE). */
+
+/* keep[i] = 1 iff row i's residue run (runs[i], rows with equal (resid,
+ * resnum) that are adjacent) has an atom with sqrt((d0*d0+d1*d1)+d2*d2) <
+ * radius, d = x - center, no FMA.  Replaces the decision of
+ * filter_radius_whole_residue, CPET/utils/io.py:283-338. */
+int cpet_filter_radius_residue(cpet_ctx *ctx, int N, const void *x, int x_dtype,
+                               const double *center, int sub_dtype, double radius,
+                               const int32_t *runs, int n_runs, uint8_t *keep,
+                               uint32_t flags);
+
+/* out = ((x - center) @ Tinv) - box_shift (Tinv row-major 3x3; NULL Tinv
+ * means no rotation, NULL box_shift no shift).  x - center runs in
+ * sub_dtype, the dot products (d0*T0j + d1*T1j) + d2*T2j (no FMA) in
+ * dot_dtype, the shift in shift_dtype; out is stored in out_dtype.
+ * Replaces the box-frame steps of calculator.__init__,
+ * CPET/source/calculator.py:341-346,369-371. */
+int cpet_frame_transform(cpet_ctx *ctx, int N, const void *x, int x_dtype,
+                         const double *center, int sub_dtype, const double *Tinv,
+                         int dot_dtype, const double *box_shift, int shift_dtype,
+                         void *out, int out_dtype, uint32_t flags);
+
+/* keep[i] = 0 iff every (x - center)_k lies strictly inside (-dims_k,
+ * dims_k), compared in float64.  Replaces the decision of filter_in_box,
+ * CPET/utils/io.py:422-475. */
+int cpet_filter_in_box(cpet_ctx *ctx, int N, const void *x, int x_dtype,
+                       const double *center, int sub_dtype, const double *dims,
+                       uint8_t *keep, uint32_t flags);
+
+/* Stable compaction of the kept rows: x_out (count,3), q_out (count) and
+ * idx_out (count, original row indices; may be NULL); *count = kept rows.
+ * The np.delete / boolean-mask step of the same filters. */
+int cpet_compact_charges(cpet_ctx *ctx, int N, const uint8_t *keep, const float *x,
+                         const float *q, float *x_out, float *q_out,
+                         int32_t *idx_out, int *count, uint32_t flags);
+
 /* Opt-in per-phase timing with HIP events recorded on the context's stream
  * (the stream the kernels run on).  After a call returns (or after
  * cpet_ctx_sync for CPET_F_NO_SYNC calls), cpet_ctx_phase_times gives the

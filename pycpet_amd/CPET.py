@@ -1,0 +1,106 @@
+"""Options-file driver for the hot-path methods, mirroring the reference's
+`CPET` class (CPET/source/CPET.py:24-229).
+
+`CPET(options).run()` dispatches "topo", "topo_GPU", "volume",
+"volume_ESP", "point_field" and "point_mag" with the reference's frame loop:
+pick an unfinished input at random, skip it when its output already exists
+(so several processes can share a directory and a rerun resumes), build a
+`calculator`, compute, and write <frame>.top / <frame>_efield.dat /
+<frame>_esp.dat / point_field.dat in the reference formats.  Methods outside
+the hot path (cluster, pca, visualize, box_check) raise NotImplementedError.
+"""
+from __future__ import annotations
+
+import logging
+import os
+from random import choice
+
+import numpy as np
+
+from .calculator import calculator
+from .io import check_valid_options, default_options_initializer, get_input_files, save_numpy_as_dat
+
+log = logging.getLogger(__name__)
+
+HOT_METHODS = ("topo", "topo_GPU", "volume", "volume_ESP", "point_field", "point_mag")
+
+
+class CPET:
+    def __init__(self, options):
+        self.options = default_options_initializer(options)
+        check_valid_options(self.options)
+        self.log = log
+        self.m = self.options["CPET_method"]
+        self.inputpath = self.options["inputpath"]
+        self.outputpath = self.options["outputpath"]
+        self.inputfiletype = self.options["inputfiletype"]
+        if not os.path.exists(self.outputpath):
+            self.log.warning("Output directory does not exist, creating: %s", self.outputpath)
+            os.makedirs(self.outputpath)
+
+    def run(self):
+        if self.m == "topo":
+            self.run_topo()
+        elif self.m == "topo_GPU":
+            self.run_topo(gpu=True)
+        elif self.m == "volume":
+            self.run_volume()
+        elif self.m == "volume_ESP":
+            self.run_volume_ESP()
+        elif self.m in ("point_field", "point_mag"):
+            self.run_point()
+        else:
+            raise NotImplementedError(
+                f"Method {self.m} is not one of the accelerated methods {HOT_METHODS}")
+
+    def _frames(self, suffix):
+        """Yields (file, frame) for unfinished inputs in random order."""
+        files = get_input_files(self.inputpath, filetype=self.inputfiletype)
+        while files:
+            f = choice(files)
+            files.remove(f)
+            frame = f.split("/")[-1].split(".")[0]
+            done = [p for p in os.listdir(self.outputpath) if p.endswith(suffix)]
+            if frame + suffix in done:
+                self.log.info("Already done for protein: %s, skipping...", frame)
+                continue
+            yield f, frame
+
+    def run_topo(self, gpu=False):
+        runtype = "compute_topo_GPU_batch_filter" if gpu else "compute_topo_complete_c_shared"
+        for f, frame in self._frames(".top"):
+            self.calculator = calculator(self.options, path_to_pdb=f)
+            hist = getattr(self.calculator, runtype)()
+            np.savetxt(self.outputpath + f"/{frame}.top", hist)
+
+    def _meta(self, mesh_shape):
+        c = self.calculator
+        return {"dimensions": c.dimensions, "step_size": [c.step_size] * 3,
+                "num_steps": [mesh_shape[0], mesh_shape[1], mesh_shape[2]],
+                "transformation_matrix": c.transformation_matrix, "center": c.center}
+
+    def run_volume(self):
+        for f, frame in self._frames("_efield.dat"):
+            self.calculator = calculator(self.options, path_to_pdb=f)
+            field_box, mesh_shape = self.calculator.compute_box()
+            save_numpy_as_dat(name=self.outputpath + f"/{frame}_efield.dat", volume=field_box,
+                              meta_data=self._meta(mesh_shape))
+
+    def run_volume_ESP(self):
+        for f, frame in self._frames("_esp.dat"):
+            self.calculator = calculator(self.options, path_to_pdb=f)
+            esp_box, mesh_shape = self.calculator.compute_box_ESP()
+            save_numpy_as_dat(name=self.outputpath + f"/{frame}_esp.dat", volume=esp_box,
+                              meta_data=self._meta(mesh_shape))
+
+    def run_point(self):
+        files = get_input_files(self.inputpath, filetype=self.inputfiletype)
+        if self.m == "point_field":
+            outfile, runtype = self.outputpath + "/point_field.dat", "compute_point_field"
+        else:
+            outfile, runtype = self.outputpath + "/point_mag.dat", "compute_point_mag"
+        with open(outfile, "w") as fh:
+            for f in files:
+                self.calculator = calculator(self.options, path_to_pdb=f)
+                frame = f.split("/")[-1].split(".")[0]
+                fh.write("{}:{}\n".format(frame, getattr(self.calculator, runtype)()))

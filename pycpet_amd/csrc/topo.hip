@@ -13,8 +13,9 @@
 //   (math_module.c:108-121, 575-584), feature = [|x0 - xf|, mean kappa].
 //
 // Pipeline:
-//   1. order: counting sort of streamline ids by n_iter, longest first, so
-//      the persistent kernel's tail consists of short streamlines.
+//   1. order: counting sort of streamline ids by n_iter (4096 keys),
+//      longest first, so the persistent kernel's tail consists of short
+//      streamlines.
 //   2. propagate (persistent): a workgroup of S waves owns 64 streamline
 //      slots; wave s sums the field of charge slice s for all 64 slots (FP32
 //      per 64-charge tile, FP64 across tiles), the S partials are reduced in
@@ -32,7 +33,7 @@
 namespace cpet {
 namespace {
 
-constexpr int kNumBuckets = 65536;
+constexpr int kNumBuckets = 4096;
 
 __device__ __forceinline__ double rsqrt_f64(double x) {
   // Hardware estimate refined by two Newton-Raphson steps for 1/sqrt(x).
@@ -51,16 +52,26 @@ __global__ __launch_bounds__(256) void fill_u32_kernel(unsigned int* p,
 }
 
 // ---- 1. longest-first order -------------------------------------------
+// Keys are exact below 2048 steps and 16-step classes above (4096 keys), so
+// the per-block histograms fit in LDS; only the order of processing depends
+// on them, never a result.
 
 __device__ __forceinline__ int bucket_of(int n) {
-  const int k = n < 0 ? 0 : (n >= kNumBuckets ? kNumBuckets - 1 : n);
-  return kNumBuckets - 1 - k;  // descending n_iter
+  n = n < 0 ? 0 : n;
+  const int key = n < 2048 ? n : 2048 + min((n - 2048) >> 4, 2047);
+  return kNumBuckets - 1 - key;  // descending n_iter
 }
 
-__global__ __launch_bounds__(256) void bucket_count_kernel(
+__global__ __launch_bounds__(1024) void bucket_count_kernel(
     int L, const int32_t* __restrict__ n_iter, unsigned int* counts) {
-  const int l = blockIdx.x * blockDim.x + threadIdx.x;
-  if (l < L) atomicAdd(&counts[bucket_of(n_iter[l])], 1u);
+  __shared__ unsigned int lh[kNumBuckets];
+  for (int b = threadIdx.x; b < kNumBuckets; b += blockDim.x) lh[b] = 0u;
+  __syncthreads();
+  for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < L; l += gridDim.x * blockDim.x)
+    atomicAdd(&lh[bucket_of(n_iter[l])], 1u);
+  __syncthreads();
+  for (int b = threadIdx.x; b < kNumBuckets; b += blockDim.x)
+    if (lh[b]) atomicAdd(&counts[b], lh[b]);
 }
 
 // Exclusive scan of kNumBuckets counts by one 1024-thread block.
@@ -86,14 +97,27 @@ __global__ __launch_bounds__(1024) void bucket_scan_kernel(unsigned int* counts)
   }
 }
 
-__global__ __launch_bounds__(256) void bucket_scatter_kernel(
+// One element per thread: rank inside the block's bucket from an LDS
+// atomic, one global reservation per (block, non-empty bucket).
+__global__ __launch_bounds__(1024) void bucket_scatter_kernel(
     int L, const int32_t* __restrict__ n_iter, unsigned int* offsets,
     int32_t* __restrict__ order) {
+  __shared__ unsigned int lh[kNumBuckets];
+  __shared__ unsigned int base[kNumBuckets];
+  for (int b = threadIdx.x; b < kNumBuckets; b += blockDim.x) lh[b] = 0u;
+  __syncthreads();
   const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  int b = 0;
+  unsigned int rank = 0;
   if (l < L) {
-    const unsigned int pos = atomicAdd(&offsets[bucket_of(n_iter[l])], 1u);
-    order[pos] = l;
+    b = bucket_of(n_iter[l]);
+    rank = atomicAdd(&lh[b], 1u);
   }
+  __syncthreads();
+  for (int k = threadIdx.x; k < kNumBuckets; k += blockDim.x)
+    if (lh[k]) base[k] = atomicAdd(&offsets[k], lh[k]);
+  __syncthreads();
+  if (l < L) order[base[b] + rank] = l;
 }
 
 // ---- 2. persistent propagation ----------------------------------------
@@ -418,10 +442,10 @@ int topo_run(cpet_ctx* ctx, int N, int L, const float* starts,
   if ((rc = prof_mark(ctx, 0))) return rc;
   hipLaunchKernelGGL(fill_u32_kernel, dim3((kNumBuckets + 255) / 256), dim3(256), 0, st,
                      d_buckets, (long long)kNumBuckets, 0u);
-  hipLaunchKernelGGL(bucket_count_kernel, dim3((L + 255) / 256), dim3(256), 0, st, L,
-                     (const int32_t*)diter, d_buckets);
+  hipLaunchKernelGGL(bucket_count_kernel, dim3(std::min((L + 1023) / 1024, 256)), dim3(1024),
+                     0, st, L, (const int32_t*)diter, d_buckets);
   hipLaunchKernelGGL(bucket_scan_kernel, dim3(1), dim3(1024), 0, st, d_buckets);
-  hipLaunchKernelGGL(bucket_scatter_kernel, dim3((L + 255) / 256), dim3(256), 0, st, L,
+  hipLaunchKernelGGL(bucket_scatter_kernel, dim3((L + 1023) / 1024), dim3(1024), 0, st, L,
                      (const int32_t*)diter, d_buckets, d_order);
   CPET_HIP_TRY(hipGetLastError());
 

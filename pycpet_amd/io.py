@@ -1,0 +1,287 @@
+"""Options, input parsing, atom filters and output writers.
+
+Mirrors the hot-path parts of the reference's CPET/utils/io.py with the same
+function names, arguments, return values and error behaviour:
+
+  default_options_initializer / check_valid_options   io.py:200-266
+  get_input_files                                      io.py:10-26
+  filter_radius, filter_radius_whole_residue           io.py:269-338
+  filter_atomspec, filter_in_box                       io.py:341-475
+  parse_pqr, parse_pdb                                 io.py:478-636
+  get_atoms_for_axes, calculate_center                 io.py:639-702
+  save_numpy_as_dat                                    io.py:50-109
+
+The radius and box filter decisions can also run on the GPU
+(pycpet_amd.frame); the functions here keep the reference's numpy
+semantics so both can be compared on identical coordinates.
+"""
+from __future__ import annotations
+
+import logging
+from glob import glob
+
+import numpy as np
+
+log = logging.getLogger(__name__)
+
+ATOM_ATTRIBUTES = ["atom_number", "atom_type", "resid", "resnum", "chain"]
+
+
+def get_input_files(inputpath, filetype):
+    if filetype not in ["pdb", "pqr"]:
+        raise ValueError("filetype must be 'pdb' or 'pqr'")
+    files_input = glob(inputpath + f"/*.{filetype}")
+    if len(files_input) == 0:
+        raise ValueError("No pdb files found in the input directory")
+    if len(files_input) == 1:
+        log.warning("Only one pdb file found in the input directory")
+    return files_input
+
+
+def default_options_initializer(options):
+    """Fills missing option keys with the reference's defaults."""
+    defaults = {
+        "inputpath": "./inpdir",
+        "outputpath": "./outdir",
+        "inputfiletype": "pdb",
+        "profile": False,
+        "write_transformed_pdb": False,
+        "dimensions": None,
+        "step_size": None,
+        "initializer": "uniform",
+        "box_shift": [0, 0, 0],
+        "filter_intersect": False,
+        "concur_slip": 4,
+        "dtype": "float32",
+        "GPU_batch_freq": 100,
+        "n_samples": None,
+    }
+    for k, v in defaults.items():
+        if k not in options.keys():
+            options[k] = v
+    return options
+
+
+def check_valid_options(options):
+    if options["inputfiletype"] not in ["pdb", "pqr"]:
+        raise ValueError("inputfiletype must be 'pdb' or 'pqr'")
+    if options["initializer"] not in ["uniform", "random"]:
+        raise ValueError("initializer must be 'uniform' or 'random'")
+    if "x" in options.keys() and "y" in options.keys():
+        if options["x"] == options["y"]:
+            raise ValueError("x and y points used for vectors cannot be the same")
+
+
+# ----------------------------------------------------------------- filters
+
+def filter_radius(x, Q, center, radius=2.0):
+    """Keeps charges with |x - center| < radius."""
+    mask = np.linalg.norm(x - center, axis=1) < radius
+    x_filtered = x[mask]
+    Q_filtered = Q[mask]
+    log.info("radius filter leaves: %d", len(Q_filtered))
+    return x_filtered, Q_filtered
+
+
+def residue_runs(ID):
+    """Run index of each row: consecutive rows with equal (resid, resnum)
+    share a run (a residue); equal ids that are not adjacent start new runs."""
+    n = len(ID)
+    if n == 0:
+        return np.zeros(0, dtype=np.int32)
+    resids = np.array([row[2] for row in ID])
+    resnums = np.array([row[3] for row in ID])
+    new = np.ones(n, dtype=bool)
+    new[1:] = (resids[1:] != resids[:-1]) | (resnums[1:] != resnums[:-1])
+    return (np.cumsum(new) - 1).astype(np.int32)
+
+
+def filter_radius_whole_residue(x, Q, ID, center, radius=2.0, keep=None):
+    """Keeps whole residues (runs of equal (resid, resnum)) that have at least
+    one atom with |x - center| < radius.  `keep` may carry a precomputed
+    keep-mask (e.g. from the GPU kernel) in place of the numpy decision."""
+    if keep is None:
+        inside = np.linalg.norm(x - center, axis=1) < radius
+        runs = residue_runs(ID)
+        any_in = np.zeros(runs.max() + 1 if len(runs) else 0, dtype=bool)
+        np.logical_or.at(any_in, runs, inside)
+        keep = any_in[runs]
+    drop = np.flatnonzero(~keep)
+    x_filtered = np.delete(x, drop, axis=0)
+    Q_filtered = np.delete(Q, drop, axis=0)
+    ID_filtered = np.delete(ID, drop, axis=0)
+    log.info("radius filter leaves: %d", len(Q_filtered))
+    return x_filtered, Q_filtered, ID_filtered
+
+
+def filter_atomspec(x, Q, ID, filter_dict, intersect):
+    """Removes atoms matching the filter: union of listed attribute values,
+    or (intersect=True) an ordered intersection across keys with "*" and ""
+    as wildcards."""
+    considered = []
+    for key in filter_dict.keys():
+        if key not in ATOM_ATTRIBUTES:
+            raise ValueError(f"Key {key} not recognized. Must be one of {ATOM_ATTRIBUTES}")
+        considered.append(key)
+    drop = set()
+    if not intersect:
+        for key in considered:
+            vals = filter_dict[key]
+            col = ATOM_ATTRIBUTES.index(key)
+            for j in range(len(ID)):
+                if ID[j][col] in vals:
+                    drop.add(j)
+    else:
+        lens = [len(filter_dict[k]) for k in considered]
+        if len(set(lens)) != 1:
+            raise ValueError(
+                "All filter lists must be the same length when using intersection filtering")
+        for i in range(lens[0]):
+            want = {k: filter_dict[k][i] for k in considered}
+            for j in range(len(ID)):
+                ok = True
+                for k, v in want.items():
+                    if v != "*" and v != "" and ID[j][ATOM_ATTRIBUTES.index(k)] != v:
+                        ok = False
+                        break
+                if ok:
+                    drop.add(j)
+    drop = list(drop)
+    x_filtered = np.delete(x, drop, axis=0)
+    Q_filtered = np.delete(Q, drop, axis=0)
+    ID_filtered = np.delete(ID, drop, axis=0)
+    if len(x_filtered) != len(Q_filtered) or len(x_filtered) != len(ID_filtered):
+        raise ValueError("Unexpected: filtered arrays have different lengths")
+    log.info("Number of atoms filtered out: %d", len(drop))
+    return x_filtered, Q_filtered, ID_filtered
+
+
+def filter_in_box(x, Q, ID, center, dimensions, keep=None):
+    """Removes charges strictly inside +-dimensions of (x - center).  Note the
+    reference calls this on coordinates already centred and rotated, so
+    `center` is subtracted a second time (kept as is)."""
+    if keep is None:
+        xr = x - center
+        inside = np.ones(xr.shape[0], dtype=bool)
+        for i in range(3):
+            inside &= (xr[:, i] > -dimensions[i]) & (xr[:, i] < dimensions[i])
+        keep = ~inside
+    return x[keep], Q[keep], np.asarray(ID)[keep]
+
+
+# ------------------------------------------------------------------ parsing
+
+def parse_pqr(path_to_pqr):
+    """PQR parser with the reference's column-shift rule for residue numbers
+    above 999 and its token logic for long record names."""
+    x, Q, atom_num, res_name, res_num, atom_type = [], [], [], [], [], []
+    with open(path_to_pqr) as fh:
+        lines = fh.readlines()
+    for line in lines:
+        if not (line.startswith("ATOM") or line.startswith("HETATM")):
+            continue
+        tok = line.split()
+        shift = 0
+        res_ind = 5
+        merged = len(tok[0]) > 6  # "HETATM12345" without a separating space
+        if merged:
+            res_ind = 4
+            atom_num.append(int(tok[0][6:]))
+            res_name.append(tok[2])
+            atom_type.append(tok[1])
+        else:
+            atom_num.append(int(tok[1]))
+            res_name.append(tok[3])
+            atom_type.append(tok[2])
+        if len(tok[res_ind]) > 4:
+            res_val = int(tok[res_ind - 1][1:])
+        else:
+            res_val = int(tok[res_ind])
+        res_num.append(res_val)
+        if res_val > 999:
+            shift += int(np.log10(res_val) - 2)
+        coords = [line[30 + shift:38 + shift], line[38 + shift:46 + shift],
+                  line[46 + shift:54 + shift]]
+        charge = line[54 + shift:61 + shift].strip()
+        try:
+            tempq = float(charge)
+            temp = [float(c.strip()) for c in coords]
+        except ValueError as exc:
+            raise ValueError(
+                "Charge or coordinates is not a useable number. Check pqr file "
+                f"formatting for the following line: {line}") from exc
+        x.append(temp)
+        Q.append(tempq)
+    return (np.array(x), np.array(Q).reshape(-1, 1), np.array(atom_num),
+            np.array(res_name), np.array(res_num), np.array(atom_type))
+
+
+def parse_pdb(pdb_file_path, get_charges=False, float32=True):
+    """Fixed-column PDB parser; charges are read from columns 56-64."""
+    dtype = "float32" if float32 else "float64"
+    xyz, Q, atom_number, residue_name, residue_number, atom_type, chains = ([] for _ in range(7))
+    with open(pdb_file_path, "r") as fh:
+        for line in fh:
+            if line.startswith("ATOM") or line.startswith("HETATM"):
+                atom_number.append(int(line[6:11].strip()))
+                atom_type.append(line[12:16].strip())
+                residue_name.append(line[17:20].strip())
+                chains.append(line[21])
+                residue_number.append(int(line[22:26].strip()))
+                xyz.append([float(line[30:38].strip()), float(line[38:46].strip()),
+                            float(line[46:54].strip())])
+                if get_charges:
+                    Q.append(float(line[55:64].strip()))
+    base = (np.array(xyz, dtype=dtype),)
+    rest = (np.array(atom_number, dtype=dtype), np.array(residue_name),
+            np.array(residue_number), np.array(atom_type), np.array(chains))
+    if get_charges:
+        return base + (np.array(Q, dtype=dtype).reshape(-1, 1),) + rest
+    return base + rest
+
+
+def get_atoms_for_axes(x, atom_type, residue_number, chains, options, seltype="center"):
+    sel = options[seltype]
+    if "chains" in sel.keys():
+        if type(sel["chains"]) != list:
+            raise ValueError("chains must be a list")
+        if len(sel["chains"]) != len(sel["atoms"]):
+            raise ValueError("chains must be the same length as atoms")
+        wanted = [(k, v, chain) for atom_dict, chain in zip(sel["atoms"], sel["chains"])
+                  for k, v in atom_dict.items()]
+        return [pos for atom_res in wanted for idx, pos in enumerate(x)
+                if (atom_type[idx], residue_number[idx], chains[idx]) == atom_res]
+    wanted = set((k, v) for atom_dict in sel["atoms"] for k, v in atom_dict.items())
+    return [pos for (atype, rnum), pos in zip(zip(atom_type, residue_number), x)
+            if (atype, rnum) in wanted]
+
+
+def calculate_center(coordinates, method):
+    if method == "mean":
+        return np.mean(coordinates, axis=0)
+    elif method == "first":
+        return coordinates[0]
+    elif method == "inverse":
+        return 2 * np.mean(coordinates[1:], axis=0) - coordinates[0]
+
+
+# ------------------------------------------------------------------ writers
+
+def save_numpy_as_dat(meta_data, volume, name):
+    """Writes a field/ESP volume in the CPET .dat layout: a 7-line header
+    (sample density, box, frame, centre, basis) then rows formatted %.3f."""
+    dims = meta_data["dimensions"]
+    steps = meta_data["num_steps"]
+    T = meta_data["transformation_matrix"].transpose()
+    c = meta_data["center"]
+    header = (
+        "#Sample Density: {} {} {}; Volume: Box: {} {} {}\n".format(
+            steps[0], steps[1], steps[2], dims[0], dims[1], dims[2])
+        + "#Frame 0\n"
+        + "#Center: {} {} {}\n".format(c[0], c[1], c[2])
+        + "#Basis Matrix:\n# {} {} {}\n# {} {} {}\n# {} {} {}\n".format(
+            T[0][0], T[0][1], T[0][2], T[1][0], T[1][1], T[1][2], T[2][0], T[2][1], T[2][2])
+    )
+    with open(name, "w") as fh:
+        fh.write(header)
+        np.savetxt(fh, volume, fmt="%.3f")

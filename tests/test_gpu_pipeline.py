@@ -1,0 +1,182 @@
+"""GPU frame kernels and the end-to-end options-file path.
+
+Frame masks (residue radius filter, in-box filter) must equal numpy's
+decisions exactly on identical coordinates, including points placed on the
+radius / box boundaries; the box-frame transform must agree with numpy's
+matmul to 2 ulp (numpy's BLAS may contract to FMA, the kernel does not);
+compaction must equal boolean-mask selection.  End-to-end runs of every
+accelerated options-file method are checked against the FP64 oracle with the
+tolerances of the kernel tests.
+"""
+import numpy as np
+import pytest
+
+from pdbgen import synthetic_frame
+
+pytestmark = pytest.mark.gpu
+
+
+def _boundary_points(rng, n, center, radius):
+    d = rng.standard_normal((n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return center + d * radius  # on the sphere up to rounding
+
+
+def test_radius_residue_mask_exact(gpu_ctx):
+    from pycpet_amd import frame
+    from pycpet_amd.io import residue_runs
+
+    rng = np.random.default_rng(0)
+    for dt_x, dt_c in ((np.float32, np.float32), (np.float32, np.float64), (np.float64, np.float64)):
+        c = np.array([1.25, -0.5, 2.0], dt_c)
+        x = np.concatenate([rng.uniform(-10, 10, (3000, 3)),
+                            _boundary_points(rng, 1000, c.astype(float), 6.0)]).astype(dt_x)
+        resids = np.array(["A", "B"])[(np.arange(len(x)) // 3) % 2]
+        resnums = (np.arange(len(x)) // 6) % 50
+        ID = [(i, "C", resids[i], int(resnums[i])) for i in range(len(x))]
+        runs = residue_runs(ID)
+        got = frame.radius_residue_keep(x, c, 6.0, runs, ctx=gpu_ctx)
+        inside = np.linalg.norm(x - c, axis=1) < 6.0
+        any_in = np.zeros(runs.max() + 1, bool)
+        np.logical_or.at(any_in, runs, inside)
+        np.testing.assert_array_equal(got, any_in[runs])
+
+
+def test_in_box_mask_exact(gpu_ctx):
+    from pycpet_amd import frame
+
+    rng = np.random.default_rng(1)
+    dims = np.array([1.5, 2.0, 0.1])
+    for dt in (np.float32, np.float64):
+        x = rng.uniform(-3, 3, (5000, 3))
+        x[:300, 0] = np.where(rng.random(300) < 0.5, dims[0], -dims[0])  # on the faces
+        x[300:600, 2] = np.float32(0.1)
+        x = x.astype(dt)
+        c = np.array([0.0, 0.0, 0.0], dt)
+        got = frame.in_box_keep(x, c, dims, ctx=gpu_ctx)
+        xr = x - c
+        inside = np.ones(len(x), bool)
+        for i in range(3):
+            inside &= (xr[:, i] > -dims[i]) & (xr[:, i] < dims[i])
+        np.testing.assert_array_equal(got, ~inside)
+
+
+def test_transform_within_2ulp(gpu_ctx, oracle_mod):
+    from pycpet_amd import frame
+
+    rng = np.random.default_rng(2)
+    for dt in (np.float32, np.float64):
+        x = rng.uniform(-60, 60, (4000, 3)).astype(dt)
+        c = rng.uniform(-5, 5, 3).astype(dt)
+        T = oracle_mod.box_basis(c, c + rng.standard_normal(3).astype(dt),
+                                 c + rng.standard_normal(3).astype(dt)).astype(dt)
+        Tinv = np.linalg.inv(T)
+        got = frame.frame_transform(x, c, Tinv, np.array([0.5, 0.0, -0.25]), ctx=gpu_ctx)
+        want = (x - c) @ Tinv - np.array([0.5, 0.0, -0.25])
+        assert got.dtype == want.dtype
+        tol = 2 * np.spacing(np.abs(want).max().astype(dt)) + 2 * np.spacing(np.abs(x).max().astype(dt))
+        assert np.abs(got - want).max() <= tol
+        # no shift, no rotation
+        got = frame.frame_transform(x, c, None, None, ctx=gpu_ctx)
+        np.testing.assert_array_equal(got, x - c)
+
+
+def test_compaction_matches_mask(gpu_ctx):
+    from pycpet_amd import frame
+
+    rng = np.random.default_rng(3)
+    for n in (1, 255, 256, 257, 70000):
+        keep = rng.random(n) < 0.37
+        x = rng.standard_normal((n, 3)).astype(np.float32)
+        q = rng.standard_normal(n).astype(np.float32)
+        xo, qo, idx = frame.compact(keep, x, q, ctx=gpu_ctx)
+        np.testing.assert_array_equal(xo, x[keep])
+        np.testing.assert_array_equal(qo, q[keep])
+        np.testing.assert_array_equal(idx, np.flatnonzero(keep))
+
+
+def _options(tmp_path, method, **kw):
+    o = {"CPET_method": method, "inputpath": str(tmp_path / "in"),
+         "outputpath": str(tmp_path / "out"),
+         "center": {"method": "mean", "atoms": [{"C7": 900}, {"O9": 900}]},
+         "x": {"method": "first", "atoms": [{"O9": 900}]},
+         "y": {"method": "first", "atoms": [{"N8": 900}]},
+         "filter_intersect": True, "filter": {"resid": ["LIG"], "resnum": [900]},
+         "filter_radius": 12, "dimensions": [1.0, 1.0, 1.0], "step_size": 0.25}
+    o.update(kw)
+    return o
+
+
+def test_calculator_gpu_frame_matches_numpy(tmp_path, gpu_ctx):
+    from pycpet_amd.calculator import calculator
+
+    (tmp_path / "in").mkdir()
+    synthetic_frame(tmp_path / "in" / "f.pdb", n=400, seed=4)
+    p = str(tmp_path / "in" / "f.pdb")
+    a = calculator(_options(tmp_path, "volume", frame_backend="gpu", filter_in_box=True), p)
+    b = calculator(_options(tmp_path, "volume", frame_backend="numpy", filter_in_box=True), p)
+    np.testing.assert_array_equal(a.Q, b.Q)
+    np.testing.assert_allclose(a.x, b.x, rtol=0, atol=4 * np.spacing(np.float32(20)))
+
+
+def _rel(a, b, floor_frac=1e-3):
+    a = np.asarray(a, float).reshape(len(b), -1)
+    b = np.asarray(b, float).reshape(len(b), -1)
+    nb = np.linalg.norm(b, axis=1)
+    d = np.linalg.norm(a - b, axis=1)
+    return float((d / np.maximum(nb, floor_frac * np.sqrt(np.mean(nb ** 2)))).max())
+
+
+def test_end_to_end_options_file_methods(tmp_path, gpu_ctx, oracle_mod):
+    from pycpet_amd.CPET import CPET
+    from pycpet_amd.calculator import calculator
+
+    (tmp_path / "in").mkdir()
+    synthetic_frame(tmp_path / "in" / "frame1.pdb", n=500, seed=5)
+    f = str(tmp_path / "in" / "frame1.pdb")
+
+    # volume: E on the 9^3 grid, file layout and values
+    CPET(_options(tmp_path, "volume")).run()
+    calc = calculator(_options(tmp_path, "volume"), f)
+    pts = calc.mesh.reshape(-1, 3)
+    E64, phi64 = oracle_mod.field_f64(pts, calc.x, calc.Q, want_phi=True)
+    rows = np.loadtxt(tmp_path / "out" / "frame1_efield.dat")
+    assert rows.shape == (729, 6)
+    np.testing.assert_allclose(rows[:, :3], pts, atol=5e-4)
+    assert np.abs(rows[:, 3:] - E64).max() <= 5e-4 + 1e-6 * np.abs(E64).max()
+    field_box, shape = calc.compute_box()
+    assert field_box.dtype == np.float32 and shape == (9, 9, 9, 3)
+    assert _rel(field_box[:, 3:], E64) <= 1e-5
+
+    # volume_ESP: float16 output as the reference
+    CPET(_options(tmp_path, "volume_ESP")).run()
+    esp, _ = calc.compute_box_ESP()
+    assert esp.dtype == np.float16 and esp.shape == (729, 4)
+    np.testing.assert_allclose(esp[:, 3].astype(float), phi64, rtol=2e-3, atol=2e-3)
+    assert (tmp_path / "out" / "frame1_esp.dat").exists()
+
+    # point field at the box centre (unsoftened)
+    CPET(_options(tmp_path, "point_field")).run()
+    pc = calculator(_options(tmp_path, "point_field"), f)
+    Ep, _ = oracle_mod.field_f64(np.zeros((1, 3), np.float32), pc.x, pc.Q, soften=False)
+    assert _rel(pc.compute_point_field()[None], Ep) <= 1e-5
+    assert (tmp_path / "out" / "point_field.dat").read_text().startswith("frame1:")
+
+    # topo and topo_GPU: same kernel, input order, seeded step limits
+    for method in ("topo", "topo_GPU"):
+        opts = _options(tmp_path, method, n_samples=200, step_size=0.01,
+                        max_streamline_init="fixed_rand", dimensions=[0.5, 0.5, 0.5])
+        CPET(opts).run()
+        tc = calculator(opts, f)
+        hist = tc.compute_topo_complete_c_shared() if method == "topo" else \
+            tc.compute_topo_GPU_batch_filter()
+        assert hist.dtype == np.float32 and hist.shape == (216, 2)
+        f64, s64, e64 = oracle_mod.topo_f64(tc.random_start_points, tc.random_max_samples,
+                                            tc.step_size, tc.dimensions, tc.x, tc.Q)
+        r = tc.topo_result
+        m = (r.steps == s64) & (r.exited == e64)
+        assert m.mean() >= 0.99
+        np.testing.assert_allclose(r.feats[m], f64[m], rtol=1e-4, atol=1e-7)
+        top = np.loadtxt(tmp_path / "out" / "frame1.top")
+        assert top.shape == (216, 2)
+        (tmp_path / "out" / "frame1.top").unlink()
