@@ -149,6 +149,34 @@ int cpet_topo(cpet_ctx *ctx, int N, int L, const float *starts,
               int32_t *exited, int nb_d, const double *edges_d, int nb_c,
               const double *edges_c, uint32_t *hist, uint32_t flags);
 
+/* ---- many frames per launch (batching) ---------------------------------
+ * Per-frame sizes and scalars (n_charges, n_starts, grid_dims, h, dims)
+ * are host arrays; the concatenated per-frame data (x, q, axes, starts,
+ * n_iter) and the outputs are host or device pointers (CPET_F_DEVICE).
+ * Frame f's charges are rows [sum_{g<f} n_charges[g], ...) of x and q.
+ * No single counterpart in the reference, which loops over frames in Python
+ * (CPET/source/CPET.py:96-229) and calls the per-frame kernels above. */
+
+/* Grids: frame f has the mesh of grid_dims[3f..3f+2] axis lengths whose
+ * values follow in `axes` (x values, then y, then z, frame after frame).
+ * Outputs are concatenated in frame order (E (sum M_f, 3), phi (sum M_f)).
+ * One launch for all frames. */
+int cpet_batch_volume(cpet_ctx *ctx, int F, const int32_t *n_charges,
+                      const float *x, const float *q, const int32_t *grid_dims,
+                      const float *axes, void *E_out, void *phi_out, int soften,
+                      uint32_t flags);
+
+/* Streamlines of F frames in one persistent launch: frame f has n_starts[f]
+ * starts (concatenated in `starts` / `n_iter`), step h[f] and half-widths
+ * dims[3f..3f+2].  feats (sum L, 2), steps, exited as cpet_topo in frame
+ * order; hist (F, nb_d, nb_c) per-frame histograms over shared edges. */
+int cpet_batch_topo(cpet_ctx *ctx, int F, const int32_t *n_charges,
+                    const float *x, const float *q, const int32_t *n_starts,
+                    const float *starts, const int32_t *n_iter, const float *h,
+                    const float *dims, double *feats, int32_t *steps,
+                    int32_t *exited, int nb_d, const double *edges_d, int nb_c,
+                    const double *edges_c, uint32_t *hist, uint32_t flags);
+
 /* ---- frame preparation (filters, box-frame transform, compaction) ----
  * dtype codes: 0 = float32, 1 = float64.  `x` holds N rows of 3 values in
  * x_dtype; arithmetic on (x - center) runs in sub_dtype (numpy's promotion of
@@ -204,7 +232,9 @@ int cpet_ctx_phase_times(cpet_ctx *ctx, float *ms, int n);
 /* Tuning knobs (autotuner): 0 restores the built-in heuristic.
  *   cpet_set_tuning(kind, a, b): kind 0 = grid field (a = points per lane
  *   along z, b = charge splits), kind 1 = points field (a = points per lane,
- *   b = splits), kind 2 = topo (a = waves per workgroup, b = workgroups). */
+ *   b = splits), kind 2 = topo (a = waves per workgroup, b = workgroups),
+ *   kind 3 = batch topo (a = waves per workgroup, b = persistent
+ *   workgroups), kind 4 = batch volume (a = points per lane). */
 int cpet_set_tuning(cpet_ctx *ctx, int kind, int a, int b);
 
 #ifdef __cplusplus

@@ -73,6 +73,9 @@ SIGNATURES = {
     "cpet_set_tuning": [c_void_p, c_int, c_int, c_int],
     "cpet_ctx_set_profiling": [c_void_p, c_int],
     "cpet_ctx_phase_times": [c_void_p, P, c_int],
+    "cpet_batch_volume": [c_void_p, c_int, P, P, P, P, P, P, P, c_int, c_uint32],
+    "cpet_batch_topo": [c_void_p, c_int, P, P, P, P, P, P, P, P, P, P, P, c_int, P, c_int, P,
+                        P, c_uint32],
     "cpet_filter_radius_residue": [c_void_p, c_int, P, c_int, P, c_int, ctypes.c_double, P,
                                    c_int, P, c_uint32],
     "cpet_frame_transform": [c_void_p, c_int, P, c_int, P, c_int, P, c_int, P, c_int, P,

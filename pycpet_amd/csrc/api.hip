@@ -43,6 +43,34 @@ __global__ __launch_bounds__(256) void pack_charges_kernel(
   if (out64) out64[j] = make_double4(c.x, c.y, c.z, c.w);
 }
 
+// Batched pack: thread j -> frame f (binary search on chg_off), row j - chg_off[f].
+__global__ __launch_bounds__(256) void pack_batch_kernel(
+    int F, int total, const int32_t* __restrict__ chg_off, const int64_t* __restrict__ x_off,
+    const float* __restrict__ x, const float* __restrict__ q, float4* __restrict__ out,
+    double4* __restrict__ out64) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= total) return;
+  int lo = 0, hi = F;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (chg_off[mid] <= j)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const long long loc = j - chg_off[lo];
+  const long long n = x_off[lo + 1] - x_off[lo];
+  float4 c;
+  if (loc < n) {
+    const long long g = x_off[lo] + loc;
+    c = make_float4(x[3 * g], x[3 * g + 1], x[3 * g + 2], q[g]);
+  } else {
+    c = make_float4(kPadCoord, kPadCoord, kPadCoord, 0.f);
+  }
+  out[j] = c;
+  if (out64) out64[j] = make_double4(c.x, c.y, c.z, c.w);
+}
+
 bool finite3(const float* v) {
   return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]);
 }
@@ -129,6 +157,57 @@ int pack_charges(cpet_ctx* ctx, int N, const float* x, const float* q,
   *out = (const float4*)d;
   if (out64) *out64 = (const double4*)d64;
   *npad = np;
+  return CPET_OK;
+}
+
+int pack_charges_batch(cpet_ctx* ctx, int F, const int32_t* nch, const float* x,
+                       const float* q, bool dev, const float4** out,
+                       const double4** out64, std::vector<int32_t>& chg_off,
+                       const int32_t** d_chg_off) {
+  chg_off.assign(F + 1, 0);
+  std::vector<int64_t> x_off(F + 1, 0);
+  for (int f = 0; f < F; ++f) {
+    if (nch[f] <= 0) {
+      set_error("frame %d has no charges", f);
+      return CPET_ERR_ARG;
+    }
+    x_off[f + 1] = x_off[f] + nch[f];
+    const long long next = (long long)chg_off[f] + round_up(nch[f], kTile);
+    if (next > 0x7fffffffLL) {
+      set_error("batch too large (%lld padded charges)", next);
+      return CPET_ERR_ARG;
+    }
+    chg_off[f + 1] = (int32_t)next;
+  }
+  const int total = chg_off[F];
+  const long long nraw = x_off[F];
+  const void* dx = nullptr;
+  const void* dq = nullptr;
+  int rc = stage_in(ctx, kSlotX, x, (size_t)nraw * 3 * sizeof(float), dev, &dx);
+  if (rc) return rc;
+  rc = stage_in(ctx, kSlotQ, q, (size_t)nraw * sizeof(float), dev, &dq);
+  if (rc) return rc;
+  void* tab = nullptr;
+  const size_t tab_bytes = (size_t)(F + 1) * (sizeof(int32_t) + sizeof(int64_t)) + 256;
+  if ((rc = ctx_buf(ctx, kSlotBatchTables, tab_bytes, &tab))) return rc;
+  int64_t* d_xoff = (int64_t*)tab;
+  int32_t* d_coff = (int32_t*)(d_xoff + F + 1);
+  CPET_HIP_TRY(hipMemcpyAsync(d_xoff, x_off.data(), (F + 1) * sizeof(int64_t),
+                              hipMemcpyHostToDevice, ctx->stream));
+  CPET_HIP_TRY(hipMemcpyAsync(d_coff, chg_off.data(), (F + 1) * sizeof(int32_t),
+                              hipMemcpyHostToDevice, ctx->stream));
+  void* d = nullptr;
+  if ((rc = ctx_buf(ctx, kSlotCharges, (size_t)total * sizeof(float4), &d))) return rc;
+  void* d64 = nullptr;
+  if (out64 && (rc = ctx_buf(ctx, kSlotCharges64, (size_t)total * sizeof(double4), &d64)))
+    return rc;
+  hipLaunchKernelGGL(pack_batch_kernel, dim3((total + 255) / 256), dim3(256), 0, ctx->stream,
+                     F, total, d_coff, d_xoff, (const float*)dx, (const float*)dq, (float4*)d,
+                     (double4*)d64);
+  CPET_HIP_TRY(hipGetLastError());
+  *out = (const float4*)d;
+  if (out64) *out64 = (const double4*)d64;
+  *d_chg_off = d_coff;
   return CPET_OK;
 }
 
@@ -304,7 +383,7 @@ int cpet_ctx_phase_times(cpet_ctx* ctx, float* ms, int n) {
 }
 
 int cpet_set_tuning(cpet_ctx* ctx, int kind, int a, int b) {
-  if (!ctx || kind < 0 || kind > 2 || a < 0 || b < 0) {
+  if (!ctx || kind < 0 || kind > 4 || a < 0 || b < 0) {
     set_error("cpet_set_tuning: bad arguments");
     return CPET_ERR_ARG;
   }

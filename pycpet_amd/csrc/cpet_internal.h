@@ -69,7 +69,7 @@ struct cpet_ctx {
   hipStream_t own = nullptr;     // the context's own stream
   bool own_stream = false;
   int num_cus = 0;
-  int tune[3][2] = {{0, 0}, {0, 0}, {0, 0}};  // autotuner overrides
+  int tune[5][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};  // autotuner overrides
   std::mutex mu;
   std::vector<cpet::DevBuf> bufs;  // indexed by Slot
   // host-side pinned staging for small results
@@ -102,6 +102,8 @@ enum Slot : int {
   kSlotFrame0,       // frame-transform scratch
   kSlotFrame1,
   kSlotFrame2,
+  kSlotBatchTables,  // batch offset tables
+  kSlotBatchWork,    // batch work lists / counters
   kSlotCount
 };
 
@@ -122,6 +124,15 @@ int pack_charges(cpet_ctx* ctx, int N, const float* x, const float* q,
                  bool dev, const float4** out, const double4** out64, int* npad);
 
 inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
+
+// Packs F frames' charges (x, q concatenated, nch[f] rows each) into one
+// padded float4 buffer (and a double4 copy when out64 is non-null); each
+// frame starts at chg_off[f] (a multiple of kTile).  chg_off (F+1) is filled
+// on the host; *d_chg_off points at its device copy.
+int pack_charges_batch(cpet_ctx* ctx, int F, const int32_t* nch, const float* x,
+                       const float* q, bool dev, const float4** out,
+                       const double4** out64, std::vector<int32_t>& chg_off,
+                       const int32_t** d_chg_off);
 
 // Phase timing: mark(ctx, i) records event i when profiling is on;
 // collect(ctx, n) (after the stream has been synchronised) stores the n-1

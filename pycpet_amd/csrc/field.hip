@@ -25,6 +25,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <vector>
 
 #include "cpet_internal.h"
 
@@ -335,6 +336,90 @@ __global__ __launch_bounds__(256) void field_reduce_kernel(long long M,
   store_point<WHAT>(d, m, acc, 0);
 }
 
+// Many grids in one launch: blk[2b] = frame, blk[2b+1] = block within the
+// frame, so each workgroup reads one frame's charges (wave-uniform).
+struct BatchFieldArgs {
+  const int32_t* blk;      // (nblocks, 2)
+  const int32_t* geo;      // (F, 4): nx, ny, nz, nzc
+  const int64_t* out_off;  // (F) first output point of the frame
+  const int32_t* ax_off;   // (F) offset of the frame's nx+ny+nz axis values
+  const int32_t* chg_off;  // (F+1) padded charge offsets
+  const float* axes;
+  void* e;
+  void* phi;
+  int f64;
+};
+
+template <int K, int MODE, int WHAT>
+__global__ __launch_bounds__(256) void field_batch_kernel(const float4* __restrict__ ch,
+                                                          const double4* __restrict__ chd,
+                                                          BatchFieldArgs a) {
+  const int fr = a.blk[2 * blockIdx.x], b = a.blk[2 * blockIdx.x + 1];
+  const int nx = a.geo[4 * fr], ny = a.geo[4 * fr + 1], nz = a.geo[4 * fr + 2],
+            nzc = a.geo[4 * fr + 3];
+  const long long gid = (long long)b * blockDim.x + threadIdx.x;
+  const long long lanes = (long long)nx * ny * nzc;
+  if (gid >= lanes) return;
+  const long long line = gid / nzc;
+  const int chunk = (int)(gid - line * nzc);
+  const int i = (int)(line / ny);
+  const int j = (int)(line - (long long)i * ny);
+  const int z0 = chunk * K;
+  const float* ax = a.axes + a.ax_off[fr];
+  float px[K], py[K], pz[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    px[k] = ax[i];
+    py[k] = ax[nx + j];
+    pz[k] = ax[nx + ny + min(z0 + k, nz - 1)];
+  }
+  const int c0 = a.chg_off[fr];
+  const int ntiles = (a.chg_off[fr + 1] - c0) / kTile;
+  double acc[K][4];
+  accumulate<K, true, MODE, WHAT>(px, py, pz, ch + c0, chd ? chd + c0 : nullptr, 0, ntiles, acc);
+  FieldOut o;
+  const size_t es = a.f64 ? sizeof(double) : sizeof(float);
+  const long long off = a.out_off[fr];
+  o.e = a.e ? (void*)((char*)a.e + (size_t)off * 3 * es) : nullptr;
+  o.phi = a.phi ? (void*)((char*)a.phi + (size_t)off * es) : nullptr;
+  o.part = nullptr;
+  o.f64 = a.f64;
+  o.M = (long long)nx * ny * nz;
+  const long long base = line * nz + z0;
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    if (z0 + k < nz) store_point<WHAT>(o, base + k, acc[k], 0);
+}
+
+template <int K, int MODE, int WHAT>
+void launch_batch_t(hipStream_t st, int nblk, const float4* ch, const double4* chd,
+                    const BatchFieldArgs& a) {
+  hipLaunchKernelGGL((field_batch_kernel<K, MODE, WHAT>), dim3(nblk), dim3(256), 0, st, ch, chd,
+                     a);
+}
+
+template <int WHAT>
+bool launch_batch_w(int mode, int K, hipStream_t st, int nblk, const float4* ch,
+                    const double4* chd, const BatchFieldArgs& a) {
+#define CPET_BATCH_K(MODE_)                                                   \
+  switch (K) {                                                                \
+    case 1: launch_batch_t<1, MODE_, WHAT>(st, nblk, ch, chd, a); return true; \
+    case 2: launch_batch_t<2, MODE_, WHAT>(st, nblk, ch, chd, a); return true; \
+    case 4: launch_batch_t<4, MODE_, WHAT>(st, nblk, ch, chd, a); return true; \
+    default: return false;                                                    \
+  }
+  switch (mode) {
+    case kModeAuto: CPET_BATCH_K(kModeAuto)
+    case kModeNaive: CPET_BATCH_K(kModeNaive)
+    case kModeKahan: CPET_BATCH_K(kModeKahan)
+    case kModeF64: CPET_BATCH_K(kModeF64)
+    case kModeTiled: CPET_BATCH_K(kModeTiled)
+    case kModeMixed: CPET_BATCH_K(kModeMixed)
+  }
+#undef CPET_BATCH_K
+  return false;
+}
+
 // ---- dispatch ---------------------------------------------------------
 
 struct Launch {
@@ -517,6 +602,89 @@ int field_launch(cpet_ctx* ctx, bool grid_mode, int nx, int ny, int nz,
     CPET_HIP_TRY(hipGetLastError());
   }
   if ((prc = prof_mark(ctx, 2))) return prc;
+  return CPET_OK;
+}
+
+// Many grids in one launch (see field_batch_kernel).  Host tables: nx, ny,
+// nz per frame (geo3, F*3), charges already packed (chg_off).
+int field_batch_launch(cpet_ctx* ctx, int F, const int32_t* geo3, const float* axes_dev,
+                       const float4* ch, const double4* chd, const int32_t* d_chg_off,
+                       void* E_out, void* phi_out, int soften, uint32_t flags) {
+  const int what = what_of(E_out != nullptr, phi_out != nullptr, soften);
+  if (what < 0) {
+    set_error("E and phi together require soften=1 (volume semantics)");
+    return CPET_ERR_ARG;
+  }
+  const int mode = (int)((flags & CPET_F_PREC_MASK) >> 8);
+  if (!field_mode_valid(mode)) {
+    set_error("unknown precision mode %d", mode);
+    return CPET_ERR_ARG;
+  }
+  const bool heavy = mode != kModeAuto && mode != kModeTiled && mode != kModeNaive;
+  int K = ctx->tune[4][0] > 0 ? ctx->tune[4][0] : (heavy ? 2 : 4);
+  std::vector<int32_t> geo(4 * (size_t)F), ax_off(F), blk;
+  std::vector<int64_t> out_off(F);
+  long long pts = 0;
+  int axn = 0;
+  for (int f = 0; f < F; ++f) {
+    const int nx = geo3[3 * f], ny = geo3[3 * f + 1], nz = geo3[3 * f + 2];
+    if (nx <= 0 || ny <= 0 || nz <= 0) {
+      set_error("frame %d: grid dimensions must be positive", f);
+      return CPET_ERR_ARG;
+    }
+    const int nzc = (nz + K - 1) / K;
+    geo[4 * f] = nx;
+    geo[4 * f + 1] = ny;
+    geo[4 * f + 2] = nz;
+    geo[4 * f + 3] = nzc;
+    ax_off[f] = axn;
+    axn += nx + ny + nz;
+    out_off[f] = pts;
+    pts += (long long)nx * ny * nz;
+    const long long lanes = (long long)nx * ny * nzc;
+    for (long long b = 0; b < (lanes + 255) / 256; ++b) {
+      blk.push_back(f);
+      blk.push_back((int32_t)b);
+    }
+  }
+  const size_t o_geo = 0;
+  const size_t o_ax = (o_geo + geo.size() * 4 + 255) & ~(size_t)255;
+  const size_t o_out = (o_ax + ax_off.size() * 4 + 255) & ~(size_t)255;
+  const size_t o_blk = (o_out + out_off.size() * 8 + 255) & ~(size_t)255;
+  void* work = nullptr;
+  int rc = ctx_buf(ctx, kSlotBatchWork, o_blk + blk.size() * 4 + 256, &work);
+  if (rc) return rc;
+  char* w = (char*)work;
+  hipStream_t st = ctx->stream;
+  CPET_HIP_TRY(hipMemcpyAsync(w + o_geo, geo.data(), geo.size() * 4, hipMemcpyHostToDevice, st));
+  CPET_HIP_TRY(hipMemcpyAsync(w + o_ax, ax_off.data(), ax_off.size() * 4, hipMemcpyHostToDevice, st));
+  CPET_HIP_TRY(hipMemcpyAsync(w + o_out, out_off.data(), out_off.size() * 8, hipMemcpyHostToDevice, st));
+  CPET_HIP_TRY(hipMemcpyAsync(w + o_blk, blk.data(), blk.size() * 4, hipMemcpyHostToDevice, st));
+  BatchFieldArgs a;
+  a.blk = (const int32_t*)(w + o_blk);
+  a.geo = (const int32_t*)(w + o_geo);
+  a.out_off = (const int64_t*)(w + o_out);
+  a.ax_off = (const int32_t*)(w + o_ax);
+  a.chg_off = d_chg_off;
+  a.axes = axes_dev;
+  a.e = E_out;
+  a.phi = phi_out;
+  a.f64 = (flags & CPET_F_OUT_F64) ? 1 : 0;
+  const int nblk = (int)(blk.size() / 2);
+  if ((rc = prof_mark(ctx, 0))) return rc;
+  bool ok = false;
+  switch (what) {
+    case kWhatESoft: ok = launch_batch_w<kWhatESoft>(mode, K, st, nblk, ch, chd, a); break;
+    case kWhatPhi: ok = launch_batch_w<kWhatPhi>(mode, K, st, nblk, ch, chd, a); break;
+    case kWhatESoftPhi: ok = launch_batch_w<kWhatESoftPhi>(mode, K, st, nblk, ch, chd, a); break;
+    case kWhatEHard: ok = launch_batch_w<kWhatEHard>(mode, K, st, nblk, ch, chd, a); break;
+  }
+  if (!ok) {
+    set_error("unsupported points-per-lane %d for the batch kernel (1, 2, 4)", K);
+    return CPET_ERR_ARG;
+  }
+  CPET_HIP_TRY(hipGetLastError());
+  if ((rc = prof_mark(ctx, 1))) return rc;
   return CPET_OK;
 }
 

@@ -26,6 +26,7 @@
 //      in FP64 (FP32 finite differences at h = 1e-3 lose most digits),
 //      distance, and the 2-D histogram with numpy.histogramdd bin rules.
 
+#include <algorithm>
 #include <cmath>
 
 #include "cpet_internal.h"
@@ -164,44 +165,54 @@ __device__ __forceinline__ void slice_field(float px, float py, float pz,
 
 enum SlotState : int { kActive = 0, kNeed = 1, kDone = 2 };
 
-// The charge pointer is a separate __restrict__ parameter so the compiler
-// may read charges through the scalar cache despite the kernel's stores.
-template <int S>
-__global__ __launch_bounds__(64 * S) void topo_propagate_kernel(
-    const float4* __restrict__ ch, PropArgs a) {
-  __shared__ double part[2][S][3][64];
-  __shared__ int nsid[2][64];
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int ntiles = a.npad / kTile;
-  const int tpw = (ntiles + S - 1) / S;
-  const int t0 = min(wave * tpw, ntiles);
-  const int t1 = min(t0 + tpw, ntiles);
+// One frame's streamline set as the slot loop sees it.  Streamline indices
+// are local to the frame; outputs are indexed the same way.
+struct FrameView {
+  int L;
+  const float* starts;     // (L,3)
+  const int32_t* n_iter;   // (L)
+  const int32_t* order;    // queue position -> streamline, or null (identity)
+  double h, d0, d1, d2;
+  double* fin;             // (L,3)
+  int32_t* steps;          // (L)
+  int32_t* exited;         // (L)
+  unsigned int* counter;   // next queue position
+  int t0, t1;              // this wave's charge tiles
+};
 
-  // Slot state; identical in every wave of the workgroup.
+// Runs the workgroup's 64 slots until the frame's queue is exhausted.  Slot
+// state is identical in every wave: each wave sums its charge tiles for all
+// 64 slots, the partials are reduced in a fixed order through LDS, and all
+// waves apply the same FP64 update.  A finished slot asks the queue for the
+// next streamline (one idle iteration).  first_q >= 0 gives the slot an
+// initial queue position, first_q < 0 starts it empty.
+template <int S>
+__device__ __forceinline__ void run_slots(const float4* __restrict__ ch, const FrameView& f,
+                                          int first_q, double (*part)[S][3][64],
+                                          int (*nsid)[64], int lane, int wave) {
   int st = kDone, sid = 0, step = 0, nmax = 0;
   double px = 0.0, py = 0.0, pz = 0.0;
 
   auto finish = [&](int out) {
     if (wave == 0) {
-      a.fin[3 * sid] = px;
-      a.fin[3 * sid + 1] = py;
-      a.fin[3 * sid + 2] = pz;
-      a.steps[sid] = step;
-      a.exited[sid] = out;
+      f.fin[3 * sid] = px;
+      f.fin[3 * sid + 1] = py;
+      f.fin[3 * sid + 2] = pz;
+      f.steps[sid] = step;
+      f.exited[sid] = out;
     }
     st = kNeed;
   };
   auto load = [&](int q) {
-    if (q >= a.L) {
+    if (q >= f.L) {
       st = kDone;
       return;
     }
-    sid = a.order[q];
-    px = (double)a.starts[3 * sid];
-    py = (double)a.starts[3 * sid + 1];
-    pz = (double)a.starts[3 * sid + 2];
-    nmax = a.n_iter[sid];
+    sid = f.order ? f.order[q] : q;
+    px = (double)f.starts[3 * sid];
+    py = (double)f.starts[3 * sid + 1];
+    pz = (double)f.starts[3 * sid + 2];
+    nmax = f.n_iter[sid];
     step = 0;
     if (nmax <= 0)
       finish(0);  // the reference loop body never runs: xf = x0
@@ -209,17 +220,20 @@ __global__ __launch_bounds__(64 * S) void topo_propagate_kernel(
       st = kActive;
   };
 
-  load(blockIdx.x * 64 + lane);
+  if (first_q >= 0)
+    load(first_q);
+  else
+    st = kNeed;
   int par = 0;
   while (__ballot(st != kDone) != 0ull) {
     double fx = 0.0, fy = 0.0, fz = 0.0;
     if (st == kActive)
-      slice_field((float)px, (float)py, (float)pz, ch, t0, t1, fx, fy, fz);
+      slice_field((float)px, (float)py, (float)pz, ch, f.t0, f.t1, fx, fy, fz);
     part[par][wave][0][lane] = fx;
     part[par][wave][1][lane] = fy;
     part[par][wave][2][lane] = fz;
     if (wave == 0 && st == kNeed)
-      nsid[par][lane] = (int)atomicAdd(a.counter, 1u);
+      nsid[par][lane] = (int)atomicAdd(f.counter, 1u);
     __syncthreads();
     if (st == kActive) {
       double ex = 0.0, ey = 0.0, ez = 0.0;
@@ -233,12 +247,12 @@ __global__ __launch_bounds__(64 * S) void topo_propagate_kernel(
       ey *= kCoulomb;
       ez *= kCoulomb;
       const double n = sqrt(ex * ex + ey * ey + ez * ez);
-      px = px + a.h * ex / n;
-      py = py + a.h * ey / n;
-      pz = pz + a.h * ez / n;
+      px = px + f.h * ex / n;
+      py = py + f.h * ey / n;
+      pz = pz + f.h * ez / n;
       ++step;
-      const int out = (px < -a.d0 || px > a.d0 || py < -a.d1 || py > a.d1 ||
-                       pz < -a.d2 || pz > a.d2)
+      const int out = (px < -f.d0 || px > f.d0 || py < -f.d1 || py > f.d1 ||
+                       pz < -f.d2 || pz > f.d2)
                           ? 1
                           : 0;
       if (out || step >= nmax) finish(out);
@@ -246,6 +260,96 @@ __global__ __launch_bounds__(64 * S) void topo_propagate_kernel(
       load(nsid[par][lane]);
     }
     par ^= 1;
+  }
+}
+
+// Single frame.  The charge pointer is a separate __restrict__ parameter so
+// the compiler may read charges through the scalar cache despite the
+// kernel's stores.
+template <int S>
+__global__ __launch_bounds__(64 * S) void topo_propagate_kernel(
+    const float4* __restrict__ ch, PropArgs a) {
+  __shared__ double part[2][S][3][64];
+  __shared__ int nsid[2][64];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int ntiles = a.npad / kTile;
+  const int tpw = (ntiles + S - 1) / S;
+  FrameView f;
+  f.L = a.L;
+  f.starts = a.starts;
+  f.n_iter = a.n_iter;
+  f.order = a.order;
+  f.h = a.h;
+  f.d0 = a.d0;
+  f.d1 = a.d1;
+  f.d2 = a.d2;
+  f.fin = a.fin;
+  f.steps = a.steps;
+  f.exited = a.exited;
+  f.counter = a.counter;
+  f.t0 = min(wave * tpw, ntiles);
+  f.t1 = min(f.t0 + tpw, ntiles);
+  run_slots<S>(ch, f, blockIdx.x * 64 + lane, part, nsid, lane, wave);
+}
+
+// Many frames in one persistent launch.  Workgroups claim assignments
+// (frame ids, several per frame) from a global counter; inside a frame the
+// slots refill from that frame's own queue, so a frame's streamlines are
+// shared by whichever workgroups joined it and a late joiner just finds the
+// queue empty.  Each wave of a workgroup sums its slice of that frame's
+// charges (charge reads stay wave-uniform).
+struct BatchTopoArgs {
+  int F;
+  const int32_t* chg_off;     // (F+1) float4 offsets, multiples of kTile
+  const int32_t* sl_off;      // (F+1) streamline offsets
+  const float* starts;        // (sum L, 3)
+  const int32_t* n_iter;      // (sum L)
+  const double* hdims;        // (F,4): h, d0, d1, d2
+  double* fin;                // (sum L, 3)
+  int32_t* steps;             // (sum L)
+  int32_t* exited;            // (sum L)
+  unsigned int* sl_counter;   // (F) per-frame queue positions, start at 0
+  const int32_t* assign;      // frame id per assignment
+  int n_assign;
+  unsigned int* assign_counter;
+};
+
+template <int S>
+__global__ __launch_bounds__(64 * S) void topo_batch_kernel(const float4* __restrict__ ch,
+                                                            BatchTopoArgs a) {
+  __shared__ double part[2][S][3][64];
+  __shared__ int nsid[2][64];
+  __shared__ int cur;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  while (true) {
+    __syncthreads();  // previous assignment's LDS reads are done
+    if (threadIdx.x == 0) cur = (int)atomicAdd(a.assign_counter, 1u);
+    __syncthreads();
+    const int as = __builtin_amdgcn_readfirstlane(cur);
+    if (as >= a.n_assign) break;
+    const int fr = a.assign[as];
+    const int s0 = a.sl_off[fr];
+    const int c0 = a.chg_off[fr] / kTile;
+    const int ntiles = a.chg_off[fr + 1] / kTile - c0;
+    const int tpw = (ntiles + S - 1) / S;
+    FrameView f;
+    f.L = a.sl_off[fr + 1] - s0;
+    f.starts = a.starts + 3LL * s0;
+    f.n_iter = a.n_iter + s0;
+    f.order = nullptr;
+    f.h = a.hdims[4 * fr];
+    f.d0 = a.hdims[4 * fr + 1];
+    f.d1 = a.hdims[4 * fr + 2];
+    f.d2 = a.hdims[4 * fr + 3];
+    f.fin = a.fin + 3LL * s0;
+    f.steps = a.steps + s0;
+    f.exited = a.exited + s0;
+    f.counter = a.sl_counter + fr;
+    f.t0 = c0 + min(wave * tpw, ntiles);
+    f.t1 = c0 + min(wave * tpw + tpw, ntiles);
+    run_slots<S>(ch, f, -1, part, nsid, lane, wave);
   }
 }
 
@@ -323,6 +427,47 @@ __device__ __forceinline__ int hist_bin(const double* __restrict__ e, int nb,
   return (lo >= 1 && lo <= nb) ? lo - 1 : -1;
 }
 
+// Features (and histogram count) of streamline l of one frame.
+__device__ __forceinline__ void epilogue_one(const float4* __restrict__ ch, int npad,
+                                             const float* starts, const double* fin,
+                                             double h, double* feats, int l,
+                                             const EpiArgs& a, unsigned int* lh,
+                                             unsigned int* hist) {
+  double p[2][3], q1[2][3], q2[2][3];
+  for (int i = 0; i < 3; ++i) {
+    p[0][i] = (double)starts[3 * l + i];
+    p[1][i] = fin[3 * l + i];
+  }
+  step2_f64(p, q1, ch, npad, h);
+  step2_f64(q1, q2, ch, npad, h);
+  double kap[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    double v1[3], v2[3];
+    for (int i = 0; i < 3; ++i) {
+      v1[i] = q1[k][i] - p[k][i];
+      v2[i] = (q2[k][i] - 2.0 * q1[k][i]) + p[k][i];
+    }
+    kap[k] = curve_f64(v1, v2);
+  }
+  const double d0 = p[0][0] - p[1][0], d1 = p[0][1] - p[1][1], d2 = p[0][2] - p[1][2];
+  const double dist = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+  const double curv = 0.5 * (kap[0] + kap[1]);
+  feats[2 * l] = dist;
+  feats[2 * l + 1] = curv;
+  if (hist) {
+    // bin the float32-rounded features, as the .top file stores them
+    const int bd = hist_bin(a.ed, a.nbd, (double)(float)dist);
+    const int bc = hist_bin(a.ec, a.nbc, (double)(float)curv);
+    if (bd >= 0 && bc >= 0) {
+      if (a.lds_hist)
+        atomicAdd(&lh[bd * a.nbc + bc], 1u);
+      else
+        atomicAdd(&hist[bd * a.nbc + bc], 1u);
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void topo_epilogue_kernel(
     const float4* __restrict__ ch, EpiArgs a) {
   extern __shared__ unsigned int lh[];
@@ -332,46 +477,39 @@ __global__ __launch_bounds__(256) void topo_epilogue_kernel(
     __syncthreads();
   }
   const int l = blockIdx.x * blockDim.x + threadIdx.x;
-  if (l < a.L) {
-    double p[2][3], q1[2][3], q2[2][3];
-    for (int i = 0; i < 3; ++i) {
-      p[0][i] = (double)a.starts[3 * l + i];
-      p[1][i] = a.fin[3 * l + i];
-    }
-    step2_f64(p, q1, ch, a.npad, a.h);
-    step2_f64(q1, q2, ch, a.npad, a.h);
-    double kap[2];
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      double v1[3], v2[3];
-      for (int i = 0; i < 3; ++i) {
-        v1[i] = q1[k][i] - p[k][i];
-        v2[i] = (q2[k][i] - 2.0 * q1[k][i]) + p[k][i];
-      }
-      kap[k] = curve_f64(v1, v2);
-    }
-    const double d0 = p[0][0] - p[1][0], d1 = p[0][1] - p[1][1],
-                 d2 = p[0][2] - p[1][2];
-    const double dist = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-    const double curv = 0.5 * (kap[0] + kap[1]);
-    a.feats[2 * l] = dist;
-    a.feats[2 * l + 1] = curv;
-    if (a.hist) {
-      // bin the float32-rounded features, as the .top file stores them
-      const int bd = hist_bin(a.ed, a.nbd, (double)(float)dist);
-      const int bc = hist_bin(a.ec, a.nbc, (double)(float)curv);
-      if (bd >= 0 && bc >= 0) {
-        if (a.lds_hist)
-          atomicAdd(&lh[bd * a.nbc + bc], 1u);
-        else
-          atomicAdd(&a.hist[bd * a.nbc + bc], 1u);
-      }
-    }
-  }
+  if (l < a.L) epilogue_one(ch, a.npad, a.starts, a.fin, a.h, a.feats, l, a, lh, a.hist);
   if (a.hist && a.lds_hist) {
     __syncthreads();
     for (int i = threadIdx.x; i < nbins; i += blockDim.x)
       if (lh[i]) atomicAdd(&a.hist[i], lh[i]);
+  }
+}
+
+// Batch epilogue: blk[2b] = frame, blk[2b+1] = block within the frame, so a
+// workgroup only ever reads one frame's charges.  Histograms are per frame
+// (hist + frame * nbins) over shared edges.
+__global__ __launch_bounds__(256) void topo_batch_epilogue_kernel(
+    const float4* __restrict__ ch, EpiArgs a, const int32_t* __restrict__ blk,
+    const int32_t* __restrict__ chg_off, const int32_t* __restrict__ sl_off,
+    const double* __restrict__ hdims) {
+  extern __shared__ unsigned int lh[];
+  const int nbins = a.nbd * a.nbc;
+  const int fr = blk[2 * blockIdx.x], b = blk[2 * blockIdx.x + 1];
+  if (a.hist && a.lds_hist) {
+    for (int i = threadIdx.x; i < nbins; i += blockDim.x) lh[i] = 0u;
+    __syncthreads();
+  }
+  const int s0 = sl_off[fr];
+  const int L = sl_off[fr + 1] - s0;
+  const int l = b * blockDim.x + threadIdx.x;
+  unsigned int* hist = a.hist ? a.hist + (size_t)fr * nbins : nullptr;
+  if (l < L)
+    epilogue_one(ch + chg_off[fr], chg_off[fr + 1] - chg_off[fr], a.starts + 3LL * s0,
+                 a.fin + 3LL * s0, hdims[4 * fr], a.feats + 2LL * s0, l, a, lh, hist);
+  if (a.hist && a.lds_hist) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < nbins; i += blockDim.x)
+      if (lh[i]) atomicAdd(&hist[i], lh[i]);
   }
 }
 
@@ -521,6 +659,169 @@ int topo_run(cpet_ctx* ctx, int N, int L, const float* starts,
   if (!dev || !(flags & CPET_F_NO_SYNC)) {
     CPET_HIP_TRY(hipStreamSynchronize(st));
     if ((rc = prof_collect(ctx, 4))) return rc;
+  }
+  return CPET_OK;
+}
+
+// Many frames in one persistent launch (see topo_batch_kernel).
+int topo_batch_run(cpet_ctx* ctx, int F, const int32_t* n_starts, const float* starts,
+                   const int32_t* n_iter, const float* h, const float* dims, const float4* ch,
+                   const int32_t* d_chg_off, const std::vector<int32_t>& chg_off, double* feats,
+                   int32_t* steps, int32_t* exited, int nb_d, const double* edges_d, int nb_c,
+                   const double* edges_c, uint32_t* hist, uint32_t flags) {
+  const bool dev = (flags & CPET_F_DEVICE) != 0;
+  hipStream_t st = ctx->stream;
+  int rc;
+  std::vector<int32_t> sl_off(F + 1, 0);
+  for (int f = 0; f < F; ++f) {
+    if (n_starts[f] <= 0) {
+      set_error("frame %d has no streamline starts", f);
+      return CPET_ERR_ARG;
+    }
+    if (!(std::isfinite(h[f]) && h[f] > 0.f) || !(dims[3 * f] > 0.f && dims[3 * f + 1] > 0.f &&
+                                                  dims[3 * f + 2] > 0.f)) {
+      set_error("frame %d: step size and box half-widths must be finite and > 0", f);
+      return CPET_ERR_ARG;
+    }
+    const long long next = (long long)sl_off[f] + n_starts[f];
+    if (next > 0x7fffffffLL) {
+      set_error("batch too large (%lld streamlines)", next);
+      return CPET_ERR_ARG;
+    }
+    sl_off[f + 1] = (int32_t)next;
+  }
+  const int Lt = sl_off[F];
+  const void* dstarts = nullptr;
+  const void* diter = nullptr;
+  if ((rc = stage_in(ctx, kSlotPts, starts, (size_t)Lt * 3 * sizeof(float), dev, &dstarts))) return rc;
+  if ((rc = stage_in(ctx, kSlotIter, n_iter, (size_t)Lt * sizeof(int32_t), dev, &diter))) return rc;
+  void* state = nullptr;
+  if ((rc = ctx_buf(ctx, kSlotTopoState, (size_t)Lt * (5 * sizeof(double) + 2 * sizeof(int32_t)),
+                    &state)))
+    return rc;
+  double* d_fin = (double*)state;
+  double* d_feats = dev ? feats : d_fin + 3 * (size_t)Lt;
+  int32_t* d_steps = (dev && steps) ? steps : (int32_t*)(d_fin + 5 * (size_t)Lt);
+  int32_t* d_exited = (dev && exited) ? exited : (int32_t*)(d_fin + 5 * (size_t)Lt) + Lt;
+
+  // assignments: frames by decreasing work, one entry per 64 streamlines
+  std::vector<int> byc(F);
+  for (int f = 0; f < F; ++f) byc[f] = f;
+  std::sort(byc.begin(), byc.end(), [&](int a, int b) {
+    return (double)n_starts[a] * (chg_off[a + 1] - chg_off[a]) >
+           (double)n_starts[b] * (chg_off[b + 1] - chg_off[b]);
+  });
+  std::vector<int32_t> assign;
+  for (int f : byc)
+    for (int r = 0; r < (n_starts[f] + 63) / 64; ++r) assign.push_back(f);
+  std::vector<int32_t> blk;
+  for (int f = 0; f < F; ++f)
+    for (int b = 0; b < (n_starts[f] + 255) / 256; ++b) {
+      blk.push_back(f);
+      blk.push_back(b);
+    }
+  std::vector<double> hd(4 * (size_t)F);
+  for (int f = 0; f < F; ++f) {
+    hd[4 * f] = (double)h[f];
+    hd[4 * f + 1] = dims[3 * f];
+    hd[4 * f + 2] = dims[3 * f + 1];
+    hd[4 * f + 3] = dims[3 * f + 2];
+  }
+  const int nbins = hist ? nb_d * nb_c : 0;
+  const size_t o_hd = 0;
+  const size_t o_sl = o_hd + hd.size() * sizeof(double);
+  const size_t o_cnt = (o_sl + sl_off.size() * sizeof(int32_t) + 255) & ~(size_t)255;
+  const size_t o_as = (o_cnt + (F + 1) * sizeof(uint32_t) + 255) & ~(size_t)255;
+  const size_t o_blk = (o_as + assign.size() * sizeof(int32_t) + 255) & ~(size_t)255;
+  const size_t o_hist = (o_blk + blk.size() * sizeof(int32_t) + 255) & ~(size_t)255;
+  const size_t o_edges = (o_hist + (size_t)F * nbins * sizeof(uint32_t) + 255) & ~(size_t)255;
+  const size_t total = o_edges + (hist ? (size_t)(nb_d + nb_c + 2) * sizeof(double) : 0) + 256;
+  void* work = nullptr;
+  if ((rc = ctx_buf(ctx, kSlotBatchWork, total, &work))) return rc;
+  char* w = (char*)work;
+  CPET_HIP_TRY(hipMemcpyAsync(w + o_hd, hd.data(), hd.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  CPET_HIP_TRY(hipMemcpyAsync(w + o_sl, sl_off.data(), sl_off.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  CPET_HIP_TRY(hipMemcpyAsync(w + o_as, assign.data(), assign.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  CPET_HIP_TRY(hipMemcpyAsync(w + o_blk, blk.data(), blk.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  unsigned int* d_cnt = (unsigned int*)(w + o_cnt);
+  hipLaunchKernelGGL(fill_u32_kernel, dim3((F + 1 + 255) / 256), dim3(256), 0, st, d_cnt,
+                     (long long)(F + 1), 0u);
+  unsigned int* d_hist = nullptr;
+  const double *d_ed = nullptr, *d_ec = nullptr;
+  if (hist) {
+    d_hist = dev ? (unsigned int*)hist : (unsigned int*)(w + o_hist);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(((long long)F * nbins + 255) / 256), dim3(256), 0,
+                       st, d_hist, (long long)F * nbins, 0u);
+    if (dev) {
+      d_ed = edges_d;
+      d_ec = edges_c;
+    } else {
+      double* e = (double*)(w + o_edges);
+      CPET_HIP_TRY(hipMemcpyAsync(e, edges_d, (nb_d + 1) * sizeof(double), hipMemcpyHostToDevice, st));
+      CPET_HIP_TRY(hipMemcpyAsync(e + nb_d + 1, edges_c, (nb_c + 1) * sizeof(double), hipMemcpyHostToDevice, st));
+      d_ed = e;
+      d_ec = e + nb_d + 1;
+    }
+  }
+
+  int S = ctx->tune[3][0] > 0 ? ctx->tune[3][0] : 8;
+  int G = ctx->tune[3][1] > 0 ? ctx->tune[3][1] : ctx->num_cus * 2;
+  G = std::max(1, std::min<int>(G, (int)assign.size()));
+  BatchTopoArgs ba;
+  ba.F = F;
+  ba.chg_off = d_chg_off;
+  ba.sl_off = (const int32_t*)(w + o_sl);
+  ba.starts = (const float*)dstarts;
+  ba.n_iter = (const int32_t*)diter;
+  ba.hdims = (const double*)(w + o_hd);
+  ba.fin = d_fin;
+  ba.steps = d_steps;
+  ba.exited = d_exited;
+  ba.sl_counter = d_cnt;
+  ba.assign = (const int32_t*)(w + o_as);
+  ba.n_assign = (int)assign.size();
+  ba.assign_counter = d_cnt + F;
+  if ((rc = prof_mark(ctx, 0))) return rc;
+  switch (S) {
+    case 1: hipLaunchKernelGGL((topo_batch_kernel<1>), dim3(G), dim3(64), 0, st, ch, ba); break;
+    case 2: hipLaunchKernelGGL((topo_batch_kernel<2>), dim3(G), dim3(128), 0, st, ch, ba); break;
+    case 4: hipLaunchKernelGGL((topo_batch_kernel<4>), dim3(G), dim3(256), 0, st, ch, ba); break;
+    case 8: hipLaunchKernelGGL((topo_batch_kernel<8>), dim3(G), dim3(512), 0, st, ch, ba); break;
+    case 16: hipLaunchKernelGGL((topo_batch_kernel<16>), dim3(G), dim3(1024), 0, st, ch, ba); break;
+    default:
+      set_error("unsupported waves per workgroup %d (1,2,4,8,16)", S);
+      return CPET_ERR_ARG;
+  }
+  CPET_HIP_TRY(hipGetLastError());
+  if ((rc = prof_mark(ctx, 1))) return rc;
+  EpiArgs ea;
+  ea.L = 0;
+  ea.npad = 0;
+  ea.starts = (const float*)dstarts;
+  ea.fin = d_fin;
+  ea.h = 0.0;
+  ea.feats = d_feats;
+  ea.ed = d_ed;
+  ea.nbd = nb_d;
+  ea.ec = d_ec;
+  ea.nbc = nb_c;
+  ea.hist = d_hist;
+  ea.lds_hist = (hist && nbins <= 8192) ? 1 : 0;
+  const size_t lds = ea.lds_hist ? (size_t)nbins * sizeof(unsigned int) : 0;
+  hipLaunchKernelGGL(topo_batch_epilogue_kernel, dim3((unsigned)(blk.size() / 2)), dim3(256), lds,
+                     st, ch, ea, (const int32_t*)(w + o_blk), d_chg_off,
+                     (const int32_t*)(w + o_sl), (const double*)(w + o_hd));
+  CPET_HIP_TRY(hipGetLastError());
+  if ((rc = prof_mark(ctx, 2))) return rc;
+  if (!dev) {
+    CPET_HIP_TRY(hipMemcpyAsync(feats, d_feats, (size_t)Lt * 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (steps) CPET_HIP_TRY(hipMemcpyAsync(steps, d_steps, (size_t)Lt * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    if (exited) CPET_HIP_TRY(hipMemcpyAsync(exited, d_exited, (size_t)Lt * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    if (hist) CPET_HIP_TRY(hipMemcpyAsync(hist, d_hist, (size_t)F * nbins * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  }
+  if (!dev || !(flags & CPET_F_NO_SYNC)) {
+    CPET_HIP_TRY(hipStreamSynchronize(st));
+    if ((rc = prof_collect(ctx, 3))) return rc;
   }
   return CPET_OK;
 }

@@ -1,0 +1,280 @@
+"""Request serving with continuous batching on one GPU.
+
+Requests (volume / volume_ESP / topo frames) arrive over time.  The server
+keeps `n_lanes` HIP streams (each with its own libcpet_hip context).  Every
+time a lane is free it takes all admitted requests that fit the HBM budget
+(and `max_batch`), copies their inputs to the device, and launches one batch
+call per request kind (cpet_batch_volume / cpet_batch_topo) followed by
+asynchronous copies of the results into pinned host memory.  Requests that
+arrive while other batches run are therefore admitted into the next free
+lane and their work overlaps on the device with the running batches.
+Latency is measured from a request's arrival to its results being in host
+memory.
+
+The "plain" mode serves the same arrivals one request at a time on one
+stream through the per-frame entry points (cpet_field_grid / cpet_topo),
+which is the baseline the batched form is compared against.
+
+No counterpart in the reference (one process handles one frame at a time,
+CPET/source/CPET.py:96-229).
+"""
+from __future__ import annotations
+
+import collections
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import _native as nat
+from . import batch as B
+from . import ops
+
+
+@dataclass
+class Request:
+    rid: int
+    kind: str                 # "volume" | "volume_ESP" | "topo"
+    data: dict                # frame arrays (numpy)
+    arrival: float = 0.0      # seconds from the start of the run
+    t_admit: float = math.nan
+    t_done: float = math.nan
+    result: object = None
+    error: Optional[str] = None
+
+    def nbytes(self) -> int:
+        """Device footprint: inputs, outputs and per-request scratch."""
+        d = self.data
+        n = int(d["q"].size)
+        b = 16 * n + 32 * n  # raw + packed charges (float4 and double4)
+        if self.kind in ("volume", "volume_ESP"):
+            m = int(np.prod([len(a) for a in d["axes"]]))
+            b += 4 * sum(len(a) for a in d["axes"]) + m * (12 if self.kind == "volume" else 4)
+        else:
+            L = int(d["starts"].shape[0])
+            b += L * (12 + 4 + 16 + 24 + 8)
+        return b
+
+
+@dataclass
+class ServeStats:
+    completed: int = 0
+    rejected: int = 0
+    wall: float = 0.0
+    latencies: List[float] = field(default_factory=list)
+    batches: int = 0
+    max_inflight_bytes: int = 0
+
+    def summary(self) -> Dict[str, float]:
+        lat = np.array(self.latencies) if self.latencies else np.array([math.nan])
+        return {
+            "completed": self.completed,
+            "rejected": self.rejected,
+            "frames_per_s": self.completed / self.wall if self.wall > 0 else 0.0,
+            "p50_ms": float(np.percentile(lat, 50) * 1e3),
+            "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "mean_ms": float(lat.mean() * 1e3),
+            "batches": self.batches,
+            "wall_s": self.wall,
+            "max_inflight_bytes": self.max_inflight_bytes,
+        }
+
+
+class _Lane:
+    def __init__(self, device: int):
+        import torch
+
+        self.torch = torch
+        self.stream = torch.cuda.Stream(device=device)  # default priority
+        self.ctx = nat.Context(device)
+        self.reqs: List[Request] = []
+        self.event = None
+        self.outputs = []
+
+    @property
+    def busy(self) -> bool:
+        return bool(self.reqs)
+
+    def done(self) -> bool:
+        return self.event is not None and self.event.query()
+
+
+class Server:
+    def __init__(self, device: int = 0, n_lanes: int = 4, hbm_budget: int = 8 << 30,
+                 max_batch: int = 64):
+        import torch
+
+        self.torch = torch
+        self.dev = torch.device(f"cuda:{device}")
+        self.lanes = [_Lane(device) for _ in range(n_lanes)]
+        self.budget = hbm_budget
+        self.max_batch = max_batch
+        self.inflight = 0
+
+    # -------------------------------------------------------------- batching
+    def _launch(self, lane: _Lane, reqs: List[Request]) -> None:
+        torch = self.torch
+        T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(self.dev, non_blocking=True)
+        lane.reqs = reqs
+        lane.outputs = []
+        with torch.cuda.stream(lane.stream):
+            vol = [r for r in reqs if r.kind == "volume"]
+            esp = [r for r in reqs if r.kind == "volume_ESP"]
+            top = [r for r in reqs if r.kind == "topo"]
+            if vol:
+                outs = B.batch_volume([{"x": T(r.data["x"]), "q": T(r.data["q"]),
+                                        "axes": tuple(T(a) for a in r.data["axes"])} for r in vol],
+                                      want_E=True, ctx=lane.ctx)
+                for r, (E, _) in zip(vol, outs):
+                    h = torch.empty(E.shape, dtype=E.dtype, pin_memory=True)
+                    h.copy_(E, non_blocking=True)
+                    lane.outputs.append((r, h))
+            if esp:
+                outs = B.batch_volume([{"x": T(r.data["x"]), "q": T(r.data["q"]),
+                                        "axes": tuple(T(a) for a in r.data["axes"])} for r in esp],
+                                      want_E=False, want_phi=True, ctx=lane.ctx)
+                for r, (_, phi) in zip(esp, outs):
+                    h = torch.empty(phi.shape, dtype=phi.dtype, pin_memory=True)
+                    h.copy_(phi, non_blocking=True)
+                    lane.outputs.append((r, h))
+            if top:
+                outs = B.batch_topo([{"x": T(r.data["x"]), "q": T(r.data["q"]),
+                                      "starts": T(r.data["starts"]), "n_iter": T(r.data["n_iter"]),
+                                      "h": r.data["h"], "dims": r.data["dims"]} for r in top],
+                                    ctx=lane.ctx)
+                for r, res in zip(top, outs):
+                    h = torch.empty(res.feats.shape, dtype=res.feats.dtype, pin_memory=True)
+                    h.copy_(res.feats, non_blocking=True)
+                    lane.outputs.append((r, h))
+            lane.event = torch.cuda.Event()
+            lane.event.record(lane.stream)
+
+    def _finish(self, lane: _Lane, now: float, stats: ServeStats) -> None:
+        for r, h in lane.outputs:
+            r.result = h.numpy()
+            r.t_done = now
+            stats.latencies.append(r.t_done - r.arrival)
+            stats.completed += 1
+            self.inflight -= r.nbytes()
+        lane.reqs = []
+        lane.outputs = []
+        lane.event = None
+
+    def run(self, requests: List[Request], stats: Optional[ServeStats] = None,
+            poll_s: float = 2e-4) -> ServeStats:
+        """Serves `requests` (sorted by arrival); returns when all are done."""
+        stats = stats or ServeStats()
+        arrivals = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        pending: collections.deque = collections.deque()
+        t0 = time.perf_counter()
+        remaining = len(requests)
+        while remaining > 0:
+            now = time.perf_counter() - t0
+            while arrivals and arrivals[0].arrival <= now:
+                r = arrivals.popleft()
+                if r.nbytes() > self.budget:
+                    r.error = "request exceeds the HBM budget"
+                    stats.rejected += 1
+                    remaining -= 1
+                    continue
+                pending.append(r)
+            for lane in self.lanes:
+                if lane.busy and lane.done():
+                    n = len(lane.reqs)
+                    self._finish(lane, time.perf_counter() - t0, stats)
+                    remaining -= n
+            for lane in self.lanes:
+                if lane.busy or not pending:
+                    continue
+                take = []
+                while pending and len(take) < self.max_batch and \
+                        self.inflight + pending[0].nbytes() <= self.budget:
+                    r = pending.popleft()
+                    r.t_admit = time.perf_counter() - t0
+                    self.inflight += r.nbytes()
+                    take.append(r)
+                if take:
+                    stats.max_inflight_bytes = max(stats.max_inflight_bytes, self.inflight)
+                    self._launch(lane, take)
+                    stats.batches += 1
+            if not any(l.busy for l in self.lanes) and not pending and arrivals:
+                time.sleep(max(0.0, min(arrivals[0].arrival - now, 0.01)))
+            else:
+                time.sleep(poll_s)
+        stats.wall = time.perf_counter() - t0
+        return stats
+
+
+def serve_plain(requests: List[Request], device: int = 0,
+                stats: Optional[ServeStats] = None) -> ServeStats:
+    """One request at a time, per-frame entry points, one stream."""
+    import torch
+
+    stats = stats or ServeStats()
+    dev = torch.device(f"cuda:{device}")
+    ctx = nat.Context(device)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    queue = collections.deque(sorted(requests, key=lambda r: r.arrival))
+    t0 = time.perf_counter()
+    while queue:
+        r = queue.popleft()
+        now = time.perf_counter() - t0
+        if r.arrival > now:
+            time.sleep(r.arrival - now)
+        r.t_admit = time.perf_counter() - t0
+        d = r.data
+        if r.kind in ("volume", "volume_ESP"):
+            E, phi = ops.field_grid(*(T(a) for a in d["axes"]), T(d["x"]), T(d["q"]),
+                                    want_E=r.kind == "volume", want_phi=r.kind == "volume_ESP",
+                                    ctx=ctx)
+            out = E if r.kind == "volume" else phi
+        else:
+            res = ops.topo(T(d["starts"]), T(d["n_iter"]), d["h"], T(d["dims"]), T(d["x"]),
+                           T(d["q"]), ctx=ctx)
+            out = res.feats
+        r.result = out.cpu().numpy()
+        r.t_done = time.perf_counter() - t0
+        stats.latencies.append(r.t_done - r.arrival)
+        stats.completed += 1
+    stats.wall = time.perf_counter() - t0
+    return stats
+
+
+def poisson_requests(rate: float, duration: float, seed: int = 0, *, mix=(0.5, 0.2, 0.3),
+                     n_range=(1000, 60000), edge_range=(21, 101), starts_range=(10000, 200000),
+                     topo_h: float = 0.05, topo_box: float = 0.5, topo_cap: int = 40,
+                     max_requests: Optional[int] = None) -> List[Request]:
+    """Poisson arrivals with the serving mix: volume / volume_ESP / topo, atom
+    counts log-uniform in n_range, odd grid edges uniform in edge_range
+    (step 0.1 A), topo starts uniform in starts_range.  The topo step, box and
+    step cap are chosen so the offered load is sustainable on one GPU."""
+    from . import workloads as W
+
+    rng = np.random.default_rng(seed)
+    base_x, base_q = W.clustered(n_range[1] // 10 + 1, 10, 45.0, 1.5, -3.0, seed)
+    t, reqs = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration or (max_requests and len(reqs) >= max_requests):
+            break
+        u = rng.random()
+        kind = "volume" if u < mix[0] else ("volume_ESP" if u < mix[0] + mix[1] else "topo")
+        n = int(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]))))
+        off = int(rng.integers(0, base_q.size - n))
+        x = (base_x[off:off + n] + 0.3 * rng.standard_normal((n, 3))).astype(np.float32)
+        q = base_q[off:off + n].copy()
+        if kind == "topo":
+            L = int(rng.integers(starts_range[0], starts_range[1] + 1))
+            dims = np.full(3, topo_box, np.float32)
+            st, it, _ = W.topo_starts(L, dims, topo_h, topo_cap, seed=int(rng.integers(1 << 30)),
+                                      start_seed=int(rng.integers(1 << 30)))
+            data = {"x": x, "q": q, "starts": st, "n_iter": it, "h": topo_h, "dims": dims}
+        else:
+            e = int(rng.integers(edge_range[0] // 2, edge_range[1] // 2 + 1)) * 2 + 1
+            d = (e - 1) * 0.05
+            ax = np.linspace(-d, d, e).astype(np.float32)
+            data = {"x": x, "q": q, "axes": (ax, ax, ax)}
+        reqs.append(Request(len(reqs), kind, data, arrival=t))
+    return reqs

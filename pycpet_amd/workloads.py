@@ -170,3 +170,32 @@ def small_topo(N: int = 300, L: int = 256, cap: int = 400, seed: int = 5,
     d = np.asarray(dims, dtype=np.float32)
     starts, n_iter, ms = topo_starts(L, d, h, cap, seed=seed + 42, start_seed=seed)
     return TopoCase(f"small_topo_{N}x{L}", x, q, starts, n_iter, h, d, ms)
+
+
+def md_frames(n_frames: int = 512, seed: int = 3, n_min: int = 3000, n_max: int = 40000,
+              edges=(21, 41, 61, 81), topo_starts_per_frame: int = 20000, h: float = 0.001,
+              cap: int = 2000, topo_box: float = 1.5, jitter: float = 0.3):
+    """Mixed MD-like frame set: atom counts uniform in [n_min, n_max], the
+    clustered recipe of volume_30k with per-frame Gaussian jitter; even
+    frames are volume jobs on a grid of edge drawn from `edges` (step 0.1 A,
+    so the box half-width is (edge-1)*0.05 A), odd frames are streamline
+    jobs with `topo_starts_per_frame` starts in +-topo_box."""
+    rng = np.random.default_rng(seed)
+    base_x, base_q = clustered(n_max // 10, 10, 45.0, 1.5, -3.0, seed)
+    frames = []
+    for i in range(n_frames):
+        n = int(rng.integers(n_min, n_max + 1)) // 10 * 10
+        x = (base_x[:n] + jitter * rng.standard_normal((n, 3))).astype(np.float32)
+        q = base_q[:n].copy()
+        if i % 2 == 0:
+            e = int(rng.choice(edges))
+            d = (e - 1) * 0.05
+            ax = np.linspace(-d, d, e).astype(np.float32)
+            frames.append({"kind": "volume", "x": x, "q": q, "axes": (ax, ax, ax)})
+        else:
+            dims = np.full(3, topo_box, np.float32)
+            st, it, _ = topo_starts(topo_starts_per_frame, dims, h, cap, seed=seed * 1000 + i,
+                                    start_seed=seed * 1000 + i + 1)
+            frames.append({"kind": "topo", "x": x, "q": q, "starts": st, "n_iter": it, "h": h,
+                           "dims": dims})
+    return frames
