@@ -1,0 +1,157 @@
+"""Frame-rate benchmarks of the batched and served forms on one GPU.
+
+    python -m pycpet_amd.framebench batch --frames 64
+    python -m pycpet_amd.framebench serve --rate 200 --duration 10
+
+batch: the mixed MD frame set (pycpet_amd.workloads.md_frames: half volume
+grids of edge 21/41/61/81, half 20,000-start streamline jobs, 3k-40k atoms
+per frame), inputs resident in HBM, timed three ways:
+  plain     one launch per frame, the simple kernel shape: one lane per grid
+            point / streamline, one wave per workgroup, no charge split, no
+            slot refill;
+  per-frame one launch per frame with the tuned single-frame kernels;
+  batched   all volume frames in one launch and all streamline frames in one
+            persistent launch, on two streams.
+serve: Poisson arrivals (pycpet_amd.serve.poisson_requests) served with
+continuous batching vs one request at a time; p50/p99 latency and frames/s.
+Prints one JSON object.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+
+def _frames_to_device(frames, torch, dev):
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    out = []
+    for f in frames:
+        g = {"kind": f["kind"], "x": T(f["x"]), "q": T(f["q"])}
+        if f["kind"] == "volume":
+            g["axes"] = tuple(T(a) for a in f["axes"])
+        else:
+            g.update(starts=T(f["starts"]), n_iter=T(f["n_iter"]), h=f["h"], dims=f["dims"],
+                     dims_t=T(f["dims"]))
+        out.append(g)
+    return out
+
+
+def run_batch(n_frames: int, topo_starts: int, seed: int = 3) -> dict:
+    import torch
+
+    from . import _native as nat, batch as B, ops, workloads as W
+
+    dev = torch.device("cuda:0")
+    frames = W.md_frames(n_frames, seed=seed, topo_starts_per_frame=topo_starts)
+    dframes = _frames_to_device(frames, torch, dev)
+    vol = [f for f in dframes if f["kind"] == "volume"]
+    top = [f for f in dframes if f["kind"] == "topo"]
+    ctx = nat.Context(0)
+
+    def per_frame(plain: bool):
+        for f in dframes:
+            if f["kind"] == "volume":
+                if plain:
+                    ctx.set_tuning(0, 1, 1)
+                ops.field_grid(*f["axes"], f["x"], f["q"], ctx=ctx)
+            else:
+                if plain:
+                    ctx.set_tuning(2, 1, (f["starts"].shape[0] + 63) // 64)
+                ops.topo(f["starts"], f["n_iter"], f["h"], f["dims_t"], f["x"], f["q"], ctx=ctx)
+            torch.cuda.synchronize()
+        ctx.set_tuning(0, 0, 0)
+        ctx.set_tuning(2, 0, 0)
+
+    ctx2 = nat.Context(0)
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    results = {}
+
+    def batched():
+        with torch.cuda.stream(s1):
+            results["vol"] = B.batch_volume(vol, ctx=ctx)
+        with torch.cuda.stream(s2):
+            results["top"] = B.batch_topo(top, ctx=ctx2)
+        torch.cuda.synchronize()
+
+    # warm-up (allocations, code objects)
+    small = [f for f in dframes[:2]]
+    B.batch_volume([f for f in small if f["kind"] == "volume"], ctx=ctx)
+    B.batch_topo([f for f in small if f["kind"] == "topo"], ctx=ctx2)
+    torch.cuda.synchronize()
+
+    timings = {}
+    for name, fn in (("batched", batched), ("per_frame", lambda: per_frame(False)),
+                     ("plain", lambda: per_frame(True))):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        timings[name] = time.perf_counter() - t0
+        print(f"{name}: {timings[name]:.3f} s", file=sys.stderr, flush=True)
+    steps = sum(int(r.steps.sum().item()) for r in results["top"])
+    vpairs = sum(int(np.prod([len(a) for a in f["axes"]])) * f["q"].numel() for f in vol)
+    tpairs = (steps + 4 * sum(f["starts"].shape[0] for f in top)) * 0  # filled below
+    tpairs = sum((int(r.steps.sum().item()) + 4 * r.steps.numel()) * f["q"].numel()
+                 for r, f in zip(results["top"], top))
+    out = {"workload": "md_frames", "frames": n_frames, "volume_frames": len(vol),
+           "topo_frames": len(top), "topo_starts_per_frame": topo_starts,
+           "pair_evals": vpairs + tpairs}
+    for k, v in timings.items():
+        out[f"{k}_s"] = v
+        out[f"{k}_frames_per_s"] = n_frames / v
+    out["batched_vs_plain"] = timings["plain"] / timings["batched"]
+    out["batched_vs_per_frame"] = timings["per_frame"] / timings["batched"]
+    out["batched_pair_evals_per_s"] = (vpairs + tpairs) / timings["batched"]
+    return out
+
+
+def run_serve(rate: float, duration: float, lanes: int, seed: int = 0) -> dict:
+    import torch
+
+    from . import serve
+
+    reqs = serve.poisson_requests(rate, duration, seed=seed)
+    warm = serve.poisson_requests(50.0, 0.2, seed=seed + 1, max_requests=6)
+    srv = serve.Server(n_lanes=lanes, hbm_budget=64 << 30)
+    srv.run(warm)
+    st = srv.run(reqs).summary()
+    print(f"continuous batching: {st}", file=sys.stderr, flush=True)
+    for r in reqs:
+        r.result = None
+    torch.cuda.synchronize()
+    pl = serve.serve_plain(reqs).summary()
+    print(f"plain: {pl}", file=sys.stderr, flush=True)
+    kinds = {k: sum(r.kind == k for r in reqs) for k in ("volume", "volume_ESP", "topo")}
+    return {"workload": "poisson_serving", "rate_req_per_s": rate, "duration_s": duration,
+            "requests": len(reqs), "mix": kinds, "lanes": lanes,
+            "offered_frames_per_s": len(reqs) / duration,
+            "continuous_batching": st, "plain_one_at_a_time": pl,
+            "throughput_ratio": st["frames_per_s"] / max(pl["frames_per_s"], 1e-9),
+            "p99_ratio_plain_over_batched": pl["p99_ms"] / max(st["p99_ms"], 1e-9)}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    b = sub.add_parser("batch")
+    b.add_argument("--frames", type=int, default=64)
+    b.add_argument("--topo-starts", type=int, default=20000)
+    s = sub.add_parser("serve")
+    s.add_argument("--rate", type=float, default=200.0)
+    s.add_argument("--duration", type=float, default=10.0)
+    s.add_argument("--lanes", type=int, default=4)
+    a = ap.parse_args(argv)
+    if a.cmd == "batch":
+        out = run_batch(a.frames, a.topo_starts)
+    else:
+        out = run_serve(a.rate, a.duration, a.lanes)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -244,16 +244,21 @@ def serve_plain(requests: List[Request], device: int = 0,
 
 def poisson_requests(rate: float, duration: float, seed: int = 0, *, mix=(0.5, 0.2, 0.3),
                      n_range=(1000, 60000), edge_range=(21, 101), starts_range=(10000, 200000),
-                     topo_h: float = 0.05, topo_box: float = 0.5, topo_cap: int = 40,
+                     topo_h: float = 0.1, topo_box: float = 0.5, topo_cap: Optional[int] = None,
                      max_requests: Optional[int] = None) -> List[Request]:
     """Poisson arrivals with the serving mix: volume / volume_ESP / topo, atom
     counts log-uniform in n_range, odd grid edges uniform in edge_range
-    (step 0.1 A), topo starts uniform in starts_range.  The topo step, box and
-    step cap are chosen so the offered load is sustainable on one GPU."""
+    (step 0.1 A), topo starts uniform in starts_range.  Topo requests use
+    h = 0.1 A in a +-0.5 A box (reference step-limit rule: at most 16 steps)
+    so the offered load stays below one GPU's capacity.  Request arrays are
+    views into shared seeded base arrays (no per-request copies)."""
     from . import workloads as W
 
     rng = np.random.default_rng(seed)
     base_x, base_q = W.clustered(n_range[1] // 10 + 1, 10, 45.0, 1.5, -3.0, seed)
+    dims = np.full(3, topo_box, np.float32)
+    base_st, base_it, _ = W.topo_starts(starts_range[1], dims, topo_h, topo_cap, seed=seed + 42,
+                                        start_seed=seed + 1)
     t, reqs = 0.0, []
     while True:
         t += rng.exponential(1.0 / rate)
@@ -263,14 +268,12 @@ def poisson_requests(rate: float, duration: float, seed: int = 0, *, mix=(0.5, 0
         kind = "volume" if u < mix[0] else ("volume_ESP" if u < mix[0] + mix[1] else "topo")
         n = int(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]))))
         off = int(rng.integers(0, base_q.size - n))
-        x = (base_x[off:off + n] + 0.3 * rng.standard_normal((n, 3))).astype(np.float32)
-        q = base_q[off:off + n].copy()
+        x, q = base_x[off:off + n], base_q[off:off + n]
         if kind == "topo":
             L = int(rng.integers(starts_range[0], starts_range[1] + 1))
-            dims = np.full(3, topo_box, np.float32)
-            st, it, _ = W.topo_starts(L, dims, topo_h, topo_cap, seed=int(rng.integers(1 << 30)),
-                                      start_seed=int(rng.integers(1 << 30)))
-            data = {"x": x, "q": q, "starts": st, "n_iter": it, "h": topo_h, "dims": dims}
+            so = int(rng.integers(0, starts_range[1] - L + 1))
+            data = {"x": x, "q": q, "starts": base_st[so:so + L], "n_iter": base_it[so:so + L],
+                    "h": topo_h, "dims": dims}
         else:
             e = int(rng.integers(edge_range[0] // 2, edge_range[1] // 2 + 1)) * 2 + 1
             d = (e - 1) * 0.05
