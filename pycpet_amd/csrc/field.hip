@@ -96,7 +96,7 @@ __device__ __forceinline__ void accumulate(
   constexpr bool full64 = MODE == kModeF64;
   constexpr bool naive = MODE == kModeNaive;
   constexpr bool kahan = MODE == kModeKahan;
-  constexpr int kUnroll = (need64 || kahan) ? 4 : 16;
+  constexpr int kGroup = (need64 || kahan) ? 4 : 8;
 
   const float rclamp = __builtin_amdgcn_rsqf(1e-6f);
   const double rclampd = rsqrt_f64((double)1e-6f);
@@ -122,14 +122,23 @@ __device__ __forceinline__ void accumulate(
 #pragma unroll
       for (int c = 0; c < 4; ++c) ta[k][c] = tc[k][c] = 0.f;
     const int jb = t * kTile;
-#pragma unroll kUnroll
-    for (int jj = 0; jj < kTile; ++jj) {
-      const int j = jb + jj;
-      const float4 c = ch[j];
+    // Charges are fetched kGroup at a time (the scalar loads of a group are
+    // issued together), so the scalar-cache latency is paid once per group.
+    for (int g = 0; g < kTile; g += kGroup) {
+      float4 cg[kGroup];
+      double4 cdg[need64 ? kGroup : 1];
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u) {
+        cg[u] = ch[jb + g + u];
+        if constexpr (need64) cdg[u] = chd[jb + g + u];
+      }
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) {
+      const float4 c = cg[u];
       float sdx = 0.f, sdy = 0.f, sdxy2 = 0.f;
       double sdxd = 0.0, sdyd = 0.0, sdxy2d = 0.0;
       double4 cd = make_double4(0.0, 0.0, 0.0, 0.0);
-      if constexpr (need64) cd = chd[j];
+      if constexpr (need64) cd = cdg[u];
       if constexpr (SHARED_XY && need32) {
         sdx = px[0] - c.x;
         sdy = py[0] - c.y;
@@ -192,6 +201,7 @@ __device__ __forceinline__ void accumulate(
           }
         }
       }
+    }
     }
     if constexpr (!naive && need32) {
 #pragma unroll

@@ -142,20 +142,27 @@ __device__ __forceinline__ void slice_field(float px, float py, float pz,
                                             const float4* __restrict__ ch,
                                             int t0, int t1, double& ex,
                                             double& ey, double& ez) {
+  // Charges are fetched 8 at a time (two 64-byte scalar loads issued
+  // together), so the scalar-cache latency is paid once per 8 charges.
+  constexpr int kGroup = 8;
   ex = ey = ez = 0.0;
   for (int t = t0; t < t1; ++t) {
     float fx = 0.f, fy = 0.f, fz = 0.f;
     const float4* cp = ch + t * kTile;
-#pragma unroll 16
-    for (int j = 0; j < kTile; ++j) {
-      const float4 c = cp[j];
-      const float dx = px - c.x, dy = py - c.y, dz = pz - c.z;
-      const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-      const float ri = __builtin_amdgcn_rsqf(r2);
-      const float s = (c.w * ri) * (ri * ri);
-      fx = fmaf(s, dx, fx);
-      fy = fmaf(s, dy, fy);
-      fz = fmaf(s, dz, fz);
+    for (int g = 0; g < kTile; g += kGroup) {
+      float4 c[kGroup];
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u) c[u] = cp[g + u];
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u) {
+        const float dx = px - c[u].x, dy = py - c[u].y, dz = pz - c[u].z;
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        const float ri = __builtin_amdgcn_rsqf(r2);
+        const float s = (c[u].w * ri) * (ri * ri);
+        fx = fmaf(s, dx, fx);
+        fy = fmaf(s, dy, fy);
+        fz = fmaf(s, dz, fz);
+      }
     }
     ex += (double)fx;
     ey += (double)fy;
@@ -267,7 +274,7 @@ __device__ __forceinline__ void run_slots(const float4* __restrict__ ch, const F
 // the compiler may read charges through the scalar cache despite the
 // kernel's stores.
 template <int S>
-__global__ __launch_bounds__(64 * S) void topo_propagate_kernel(
+__global__ __launch_bounds__(64 * S, 8) void topo_propagate_kernel(
     const float4* __restrict__ ch, PropArgs a) {
   __shared__ double part[2][S][3][64];
   __shared__ int nsid[2][64];
@@ -316,7 +323,7 @@ struct BatchTopoArgs {
 };
 
 template <int S>
-__global__ __launch_bounds__(64 * S) void topo_batch_kernel(const float4* __restrict__ ch,
+__global__ __launch_bounds__(64 * S, 8) void topo_batch_kernel(const float4* __restrict__ ch,
                                                             BatchTopoArgs a) {
   __shared__ double part[2][S][3][64];
   __shared__ int nsid[2][64];
