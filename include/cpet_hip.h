@@ -237,6 +237,36 @@ int cpet_ctx_phase_times(cpet_ctx *ctx, float *ms, int n);
  *   workgroups), kind 4 = batch volume (a = points per lane). */
 int cpet_set_tuning(cpet_ctx *ctx, int kind, int a, int b);
 
+/* ------------------------------------------------------ frame text I/O
+ * Host-only (no GPU needed).  These feed frames to the kernels and write
+ * their results, at the rate a serving loop needs. */
+
+/* Counts ATOM/HETATM records of a PDB file (CPET/utils/io.py:596-598). */
+int cpet_pdb_scan(const char *path, int *n_atoms);
+
+/* Parses the n_atoms ATOM/HETATM records of a PDB file exactly as
+ * CPET/utils/io.py:574-636 (parse_pdb) does: fixed columns serial [6:11],
+ * name [12:16], resName [17:20], chain [21], resSeq [22:26], x y z
+ * [30:38] [38:46] [46:54] and, when get_charges, the charge [55:64]; fields
+ * are whitespace-stripped, numbers parsed to correctly rounded doubles.
+ * Outputs: xyz[3*n_atoms], q[n_atoms] (get_charges), atom_number, resnum,
+ * atom_type[4*n_atoms], resname[3*n_atoms] (NUL-padded), chain[n_atoms].
+ * Returns CPET_ERR_ARG, naming the line, where Python's int()/float() would
+ * raise; CPET_ERR_ARG with a message starting "non-ASCII" when a record
+ * holds non-ASCII bytes (byte columns then differ from Python's character
+ * columns). */
+int cpet_pdb_parse(const char *path, int n_atoms, int get_charges, double *xyz, double *q,
+                   int32_t *atom_number, int32_t *resnum, char *atom_type, char *resname,
+                   char *chain);
+
+/* Writes an optional header then rows x cols values, byte-identical to
+ * numpy.savetxt(path, data, fmt) for fmt 0 = "%.3f" (the volume .dat rows,
+ * CPET/utils/io.py:97-101) and fmt 1 = "%.18e" (the .top histogram,
+ * CPET/source/CPET.py:123).  dtype 0 = float32, 1 = float64, 2 = float16.
+ * Rows are formatted by n_threads threads (0 = all cores). */
+int cpet_write_rows(const char *path, const char *header, const void *data, int dtype,
+                    int64_t rows, int cols, int fmt, int n_threads);
+
 #ifdef __cplusplus
 }
 #endif

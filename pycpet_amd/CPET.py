@@ -18,7 +18,8 @@ from random import choice
 import numpy as np
 
 from .calculator import calculator
-from .io import check_valid_options, default_options_initializer, get_input_files, save_numpy_as_dat
+from .io import (check_valid_options, default_options_initializer, get_input_files,
+                 save_numpy_as_dat, write_rows)
 
 log = logging.getLogger(__name__)
 
@@ -71,7 +72,7 @@ class CPET:
         for f, frame in self._frames(".top"):
             self.calculator = calculator(self.options, path_to_pdb=f)
             hist = getattr(self.calculator, runtype)()
-            np.savetxt(self.outputpath + f"/{frame}.top", hist)
+            write_rows(self.outputpath + f"/{frame}.top", hist, "%.18e")
 
     def _meta(self, mesh_shape):
         c = self.calculator

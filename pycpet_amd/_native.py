@@ -73,6 +73,10 @@ SIGNATURES = {
     "cpet_set_tuning": [c_void_p, c_int, c_int, c_int],
     "cpet_ctx_set_profiling": [c_void_p, c_int],
     "cpet_ctx_phase_times": [c_void_p, P, c_int],
+    "cpet_pdb_scan": [ctypes.c_char_p, P],
+    "cpet_pdb_parse": [ctypes.c_char_p, c_int, c_int, P, P, P, P, P, P, P],
+    "cpet_write_rows": [ctypes.c_char_p, ctypes.c_char_p, P, c_int, ctypes.c_int64, c_int, c_int,
+                        c_int],
     "cpet_batch_volume": [c_void_p, c_int, P, P, P, P, P, P, P, c_int, c_uint32],
     "cpet_batch_topo": [c_void_p, c_int, P, P, P, P, P, P, P, P, P, P, P, c_int, P, c_int, P,
                         P, c_uint32],

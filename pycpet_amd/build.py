@@ -2,7 +2,8 @@
 
     python -m pycpet_amd.build          # or build() from __graft_entry__
 
-Each .hip translation unit is compiled to an object in parallel and linked
+Each .hip translation unit is compiled with hipcc and each host-only .cpp
+translation unit (text I/O) with g++, in parallel, and the objects are linked
 into pycpet_amd/libcpet_hip.so.  Packed FP32 instructions are disabled for
 the device pass (-Xclang -target-feature -Xclang -packed-fp32-ops).
 """
@@ -32,6 +33,9 @@ HIPCC_FLAGS = [
 ]
 
 
+CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-Wno-unused-function"]
+
+
 def _hipcc() -> str:
     exe = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     if not Path(exe).exists():
@@ -40,10 +44,14 @@ def _hipcc() -> str:
 
 
 def _compile(src: Path, obj: Path) -> None:
-    cmd = [_hipcc(), *HIPCC_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-c", str(src), "-o", str(obj)]
+    if src.suffix == ".cpp":
+        cxx = shutil.which("g++") or "g++"
+        cmd = [cxx, *CXX_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-c", str(src), "-o", str(obj)]
+    else:
+        cmd = [_hipcc(), *HIPCC_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed for {src.name}:\n{res.stderr}")
+        raise RuntimeError(f"compile failed for {src.name}:\n{res.stderr}")
     # surface real warnings, but not the host pass's unknown-feature notice
     msgs = [ln for ln in res.stderr.splitlines() if "packed-fp32-ops" not in ln and ln.strip()]
     if msgs:
@@ -51,8 +59,8 @@ def _compile(src: Path, obj: Path) -> None:
 
 
 def build(verbose: bool = True) -> Path:
-    """Compiles every csrc/*.hip and links pycpet_amd/libcpet_hip.so."""
-    srcs = sorted(CSRC.glob("*.hip"))
+    """Compiles every csrc/*.hip and csrc/*.cpp and links pycpet_amd/libcpet_hip.so."""
+    srcs = sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
     if not srcs:
         raise RuntimeError(f"no HIP sources under {CSRC}")
     objdir = PKG / "_objs"
@@ -62,7 +70,8 @@ def build(verbose: bool = True) -> Path:
         list(ex.map(lambda so: _compile(*so), zip(srcs, objs)))
     out = PKG / LIB_NAME
     tmp = PKG / (LIB_NAME + ".tmp")
-    cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
+    cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", str(tmp),
+           *map(str, objs)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stderr}")

@@ -14,6 +14,10 @@ function names, arguments, return values and error behaviour:
 The radius and box filter decisions can also run on the GPU
 (pycpet_amd.frame); the functions here keep the reference's numpy
 semantics so both can be compared on identical coordinates.
+
+PDB parsing and the .dat / .top writers run in the native library
+(cpet_pdb_parse, cpet_write_rows: host code, no GPU needed), which
+reproduces the reference's parsed values and output bytes exactly.
 """
 from __future__ import annotations
 
@@ -216,9 +220,9 @@ def parse_pqr(path_to_pqr):
             np.array(res_name), np.array(res_num), np.array(atom_type))
 
 
-def parse_pdb(pdb_file_path, get_charges=False, float32=True):
-    """Fixed-column PDB parser; charges are read from columns 56-64."""
-    dtype = "float32" if float32 else "float64"
+def _parse_pdb_text(pdb_file_path, get_charges, dtype):
+    """Line-by-line parser for files with non-ASCII records, where Python's
+    character columns and the native parser's byte columns differ."""
     xyz, Q, atom_number, residue_name, residue_number, atom_type, chains = ([] for _ in range(7))
     with open(pdb_file_path, "r") as fh:
         for line in fh:
@@ -232,11 +236,61 @@ def parse_pdb(pdb_file_path, get_charges=False, float32=True):
                             float(line[46:54].strip())])
                 if get_charges:
                     Q.append(float(line[55:64].strip()))
-    base = (np.array(xyz, dtype=dtype),)
-    rest = (np.array(atom_number, dtype=dtype), np.array(residue_name),
+    return (np.array(xyz, dtype=dtype).reshape(-1, 3), np.array(Q, dtype=dtype).reshape(-1, 1),
+            np.array(atom_number, dtype=dtype), np.array(residue_name),
             np.array(residue_number), np.array(atom_type), np.array(chains))
+
+
+def _fixed_str(buf: np.ndarray, width: int) -> np.ndarray:
+    """NUL-padded byte records -> the unicode array np.array(list_of_str)
+    gives (item width = the longest string)."""
+    rec = buf.reshape(-1, width)
+    w = max(1, int((rec != 0).sum(axis=1).max())) if rec.shape[0] else 1
+    return buf.view(f"S{width}").reshape(-1).astype(f"U{w}")
+
+
+def parse_pdb(pdb_file_path, get_charges=False, float32=True):
+    """Fixed-column PDB parser (CPET/utils/io.py:574-636): returns
+    (xyz, [Q (N,1),] atom_number, residue_name, residue_number, atom_type,
+    chains) with the reference's dtypes.  Parsing runs in cpet_pdb_parse;
+    malformed numeric fields raise ValueError as int()/float() would."""
+    import ctypes
+
+    from . import _native as nat
+
+    dtype = "float32" if float32 else "float64"
+    lib = nat.load()
+    path = str(pdb_file_path).encode()
+    n = ctypes.c_int(0)
+    rc = lib.cpet_pdb_scan(path, ctypes.byref(n))
+    if rc != nat.CPET_OK:
+        raise OSError(nat.last_error(lib))
+    N = n.value
+    if N == 0:  # np.array([]) of each list, as the reference builds them
+        e = np.array([])
+        return ((np.array([], dtype=dtype),) + ((np.array([], dtype=dtype).reshape(-1, 1),)
+                if get_charges else ()) + (np.array([], dtype=dtype), e, e, e, e))
+    xyz = np.empty((N, 3), np.float64)
+    q = np.empty(N, np.float64)
+    anum = np.empty(N, np.int32)
+    rnum = np.empty(N, np.int32)
+    atype = np.zeros(4 * N + 1, np.uint8)
+    rname = np.zeros(3 * N + 1, np.uint8)
+    chain = np.zeros(N + 1, np.uint8)
+    ptr = lambda a: a.ctypes.data
+    rc = lib.cpet_pdb_parse(path, N, int(bool(get_charges)), ptr(xyz), ptr(q), ptr(anum),
+                            ptr(rnum), ptr(atype), ptr(rname), ptr(chain))
+    if rc != nat.CPET_OK:
+        msg = nat.last_error(lib)
+        if msg.startswith("non-ASCII"):
+            out = _parse_pdb_text(pdb_file_path, get_charges, dtype)
+            return (out[0],) + ((out[1],) if get_charges else ()) + out[2:]
+        raise ValueError(msg)
+    rest = (anum.astype(dtype), _fixed_str(rname[:3 * N], 3), rnum.astype(np.int64),
+            _fixed_str(atype[:4 * N], 4), _fixed_str(chain[:N], 1))
+    base = (xyz.astype(dtype),)
     if get_charges:
-        return base + (np.array(Q, dtype=dtype).reshape(-1, 1),) + rest
+        return base + (q.astype(dtype).reshape(-1, 1),) + rest
     return base + rest
 
 
@@ -267,9 +321,36 @@ def calculate_center(coordinates, method):
 
 # ------------------------------------------------------------------ writers
 
+_ROW_DTYPES = {np.dtype(np.float32): 0, np.dtype(np.float64): 1, np.dtype(np.float16): 2}
+
+
+def write_rows(name, data, fmt="%.3f", header=""):
+    """numpy.savetxt(name, data, fmt) for fmt "%.3f" or "%.18e" (numpy's
+    default), byte-identical, with the rows formatted by cpet_write_rows on
+    all host cores; `header` is written verbatim first."""
+    from . import _native as nat
+
+    codes = {"%.3f": 0, "%.18e": 1}
+    if fmt not in codes:
+        raise ValueError(f"unsupported row format {fmt!r}")
+    a = np.asarray(data)
+    if a.ndim == 1:
+        a = a.reshape(-1, 1)
+    if a.ndim != 2:
+        raise ValueError("Expected 1D or 2D array, got %dD array instead" % a.ndim)
+    if a.dtype not in _ROW_DTYPES:
+        a = a.astype(np.float64)
+    a = np.ascontiguousarray(a)
+    lib = nat.load()
+    nat.check(lib.cpet_write_rows(str(name).encode(), header.encode() if header else None,
+                                  a.ctypes.data, _ROW_DTYPES[a.dtype], a.shape[0],
+                                  max(1, a.shape[1]), codes[fmt], 0), lib)
+
+
 def save_numpy_as_dat(meta_data, volume, name):
-    """Writes a field/ESP volume in the CPET .dat layout: a 7-line header
-    (sample density, box, frame, centre, basis) then rows formatted %.3f."""
+    """Writes a field/ESP volume in the CPET .dat layout (CPET/utils/io.py:
+    50-109): a 7-line header (sample density, box, frame, centre, basis) then
+    rows formatted %.3f."""
     dims = meta_data["dimensions"]
     steps = meta_data["num_steps"]
     T = meta_data["transformation_matrix"].transpose()
@@ -282,6 +363,4 @@ def save_numpy_as_dat(meta_data, volume, name):
         + "#Basis Matrix:\n# {} {} {}\n# {} {} {}\n# {} {} {}\n".format(
             T[0][0], T[0][1], T[0][2], T[1][0], T[1][1], T[1][2], T[2][0], T[2][1], T[2][2])
     )
-    with open(name, "w") as fh:
-        fh.write(header)
-        np.savetxt(fh, volume, fmt="%.3f")
+    write_rows(name, volume, "%.3f", header)

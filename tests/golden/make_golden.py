@@ -5,8 +5,12 @@ them (CPET/utils/c_ops.py:250-341 and CPET/utils/calculator.py:430-475).
 
 Run in a checkout where the reference source exists:
     make -C oracle ref && python tests/golden/make_golden.py
+    python tests/golden/make_golden.py pdb      # only ref_pdb_rows.pdb
 
 Each .npz holds the seeded inputs and the reference's FP32 outputs only.
+ref_pdb_rows.pdb holds lines of the reference's own test frame
+(tests/test_files/test_large.pdb): its header, first records, a TER line
+and the last HETATM records (5-digit serials, 4-digit residue numbers).
 """
 import sys
 from pathlib import Path
@@ -52,7 +56,17 @@ def topo(ref):
                         n_iter=c.n_iter, h=np.float32(c.h), dims=c.dims, ret=ret)
 
 
+def pdb_rows():
+    src = Path("/root/reference/tests/test_files/test_large.pdb").read_text().splitlines(True)
+    ter = next(i for i, ln in enumerate(src) if ln.startswith("TER"))
+    keep = src[:14] + [src[ter]] + src[-6:]
+    (OUT / "ref_pdb_rows.pdb").write_text("".join(keep))
+
+
 if __name__ == "__main__":
+    pdb_rows()
+    if sys.argv[1:] == ["pdb"]:  # text fixture only
+        sys.exit(0)
     r = Ref()
     volume(r)
     volume_floor(r)
