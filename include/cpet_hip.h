@@ -42,7 +42,9 @@ extern "C" {
 #define CPET_F_OUT_F64 0x2u  /* field/phi outputs are double, not float */
 #define CPET_F_NO_SYNC 0x4u  /* device-pointer calls: do not wait for the stream */
 /* precision of the field kernels, bits 8..10 */
-#define CPET_F_PREC_AUTO (0u << 8)  /* E: FP32 tiled; phi: mixed (default) */
+#define CPET_F_PREC_AUTO (0u << 8)  /* E: FP32 tiled, points with strong
+                                       cancellation recomputed as MIXED;
+                                       phi: MIXED (default) */
 #define CPET_F_PREC_NAIVE (1u << 8) /* one FP32 running sum */
 #define CPET_F_PREC_KAHAN (2u << 8) /* Kahan FP32 within a 64-charge tile, FP64 across */
 #define CPET_F_PREC_F64 (3u << 8)   /* FP64 throughout */
@@ -223,11 +225,20 @@ int cpet_compact_charges(cpet_ctx *ctx, int N, const uint8_t *keep, const float 
  * cpet_ctx_sync for CPET_F_NO_SYNC calls), cpet_ctx_phase_times gives the
  * durations in ms of the last call's phases:
  *   cpet_topo:          [order, propagate, epilogue(+histogram)]
- *   cpet_field_*:       [field kernel, split reduction]
+ *   cpet_field_*:       [field kernel, split reduction, FP64 recompute]
+ *   cpet_batch_volume:  [field kernel, FP64 recompute]
  * No counterpart in the reference (its timing is time.time() around calls,
  * CPET/source/calculator.py:686,706-711). */
 int cpet_ctx_set_profiling(cpet_ctx *ctx, int on);
 int cpet_ctx_phase_times(cpet_ctx *ctx, float *ms, int n);
+
+/* Number of grid / points the last field call on this context recomputed in
+ * FP64 because the auto-precision check found strong cancellation there
+ * (u * sum|q|/r^2 > 4e-6 * |E|); 0 when that call had no check (precision
+ * modes other than auto, or no E requested).  Waits for the context's
+ * stream.  No counterpart in the reference (its sums are FP32 throughout,
+ * CPET/utils/math_module.c:405-451). */
+int cpet_ctx_refined_points(cpet_ctx *ctx, long long *n);
 
 /* Tuning knobs (autotuner): 0 restores the built-in heuristic.
  *   cpet_set_tuning(kind, a, b): kind 0 = grid field (a = points per lane

@@ -73,6 +73,7 @@ SIGNATURES = {
     "cpet_set_tuning": [c_void_p, c_int, c_int, c_int],
     "cpet_ctx_set_profiling": [c_void_p, c_int],
     "cpet_ctx_phase_times": [c_void_p, P, c_int],
+    "cpet_ctx_refined_points": [c_void_p, P],
     "cpet_pdb_scan": [ctypes.c_char_p, P],
     "cpet_pdb_parse": [ctypes.c_char_p, c_int, c_int, P, P, P, P, P, P, P],
     "cpet_write_rows": [ctypes.c_char_p, ctypes.c_char_p, P, c_int, ctypes.c_int64, c_int, c_int,
@@ -188,6 +189,12 @@ class Context:
         buf = (ctypes.c_float * n)()
         check(self.lib.cpet_ctx_phase_times(self.handle, buf, n), self.lib)
         return list(buf)
+
+    def refined_points(self) -> int:
+        """Points the last auto-precision field call recomputed in FP64."""
+        n = ctypes.c_longlong(0)
+        check(self.lib.cpet_ctx_refined_points(self.handle, ctypes.byref(n)), self.lib)
+        return n.value
 
     def close(self) -> None:
         if getattr(self, "handle", None):

@@ -382,6 +382,23 @@ int cpet_ctx_phase_times(cpet_ctx* ctx, float* ms, int n) {
   return CPET_OK;
 }
 
+int cpet_ctx_refined_points(cpet_ctx* ctx, long long* n) {
+  if (!ctx || !n) {
+    set_error("cpet_ctx_refined_points: bad arguments");
+    return CPET_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  *n = 0;
+  if (!ctx->fix_count) return CPET_OK;
+  int rc = ctx_activate(ctx);
+  if (rc) return rc;
+  int v = 0;
+  CPET_HIP_TRY(hipMemcpyAsync(&v, ctx->fix_count, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  CPET_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  *n = v;
+  return CPET_OK;
+}
+
 int cpet_set_tuning(cpet_ctx* ctx, int kind, int a, int b) {
   if (!ctx || kind < 0 || kind > 4 || a < 0 || b < 0) {
     set_error("cpet_set_tuning: bad arguments");
@@ -468,7 +485,7 @@ static int field_common(cpet_ctx* ctx, bool grid_mode, int nx, int ny, int nz,
   }
   if (!dev || !(flags & CPET_F_NO_SYNC)) {
     CPET_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    if ((rc = prof_collect(ctx, 3))) return rc;
+    if ((rc = prof_collect(ctx, ctx->marks))) return rc;
   }
   return CPET_OK;
 }

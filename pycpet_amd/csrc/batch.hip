@@ -69,7 +69,7 @@ int cpet_batch_volume(cpet_ctx* ctx, int F, const int32_t* n_charges, const floa
   }
   if (!dev || !(flags & CPET_F_NO_SYNC)) {
     CPET_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    if ((rc = prof_collect(ctx, 2))) return rc;
+    if ((rc = prof_collect(ctx, ctx->marks))) return rc;
   }
   return CPET_OK;
 }

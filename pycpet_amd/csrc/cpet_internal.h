@@ -80,6 +80,9 @@ struct cpet_ctx {
   hipEvent_t ev[8] = {};
   float phase_ms[4] = {0.f, 0.f, 0.f, 0.f};
   int marks = 0;  // events recorded by the last call, not yet collected
+  // device counter of the points the last auto-mode field call recomputed
+  // in FP64 (null when that call had no check)
+  int* fix_count = nullptr;
 };
 
 namespace cpet {
@@ -104,6 +107,7 @@ enum Slot : int {
   kSlotFrame2,
   kSlotBatchTables,  // batch offset tables
   kSlotBatchWork,    // batch work lists / counters
+  kSlotFix,          // auto-mode check: counter + list of points to recompute
   kSlotCount
 };
 

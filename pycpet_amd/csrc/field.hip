@@ -14,12 +14,17 @@
 // Points kernel (arbitrary points): 3 subtracts + 3 ops for r2 instead.
 //
 // Precision is chosen per quantity (flags bits 8..10):
-//   auto   E: FP32 pair terms, FP32 sum per 64-charge tile, FP64 across tiles
+//   auto   E: FP32 pair terms, FP32 sum per 64-charge tile, FP64 across tiles,
+//          plus A = sum |q| / r^2 (the sum of the term magnitudes).  The FP32
+//          rounding error of E is at most ~0.6 u A (u = 2^-24; measured
+//          bound, the per-term roundings being independent), so a point with
+//          u A > kAutoTol |E| (strong cancellation) is appended to a list and
+//          recomputed in the "mixed" scheme by field_fix_kernel.
 //          phi: "mixed" — FP64 displacement and r^2 (exact for FP32 inputs),
 //          an FP32 rsq seed refined by one FP64 Newton step, FP64 sums.
 //          (phi cancels strongly between charges of both signs, so FP32
 //          pair rounding alone exceeds a 1e-5 pointwise bound near phi ~ 0.)
-//   tiled / kahan / naive / f64 / mixed: the same scheme for both.
+//   tiled / kahan / naive / f64 / mixed: the same scheme for both, no check.
 // Small grids split the charges over blockIdx.y and reduce the FP64 partials
 // in a fixed order in a second kernel.
 
@@ -70,6 +75,14 @@ __device__ __forceinline__ double rsqrt_refine(float seed, double x) {
   return fma(0.5 * y, e, y);
 }
 
+// Relative error target of the auto-mode check (see above): points whose
+// estimate u * A / |E| exceeds it are recomputed in FP64.
+constexpr double kAutoTol = 4e-6;
+constexpr double kUnit32 = 5.9604644775390625e-08;  // 2^-24
+
+// Slots of the per-point FP64 accumulator: E x, y, z, phi, A (auto E only).
+constexpr int kAcc = 5;
+
 __device__ __forceinline__ void kahan_add(float t, float& s, float& c) {
   const float y = t - c;
   const float u = s + y;
@@ -77,14 +90,15 @@ __device__ __forceinline__ void kahan_add(float t, float& s, float& c) {
   s = u;
 }
 
-// Accumulates the unscaled sums (sum q d / r^3 and sum q / r) of charges
-// [t0*kTile, t1*kTile) at K points into out[k][0..3] (FP64).
-// SHARED_XY: all K points share px[0], py[0] (grid z-line).
+// Accumulates the unscaled sums (sum q d / r^3, sum q / r and, for auto E,
+// sum |q| / r^2) of charges [t0*kTile, t1*kTile) at K points into
+// out[k][0..4] (FP64).  SHARED_XY: all K points share px[0], py[0] (grid
+// z-line).
 template <int K, bool SHARED_XY, int MODE, int WHAT>
 __device__ __forceinline__ void accumulate(
     const float (&px)[K], const float (&py)[K], const float (&pz)[K],
     const float4* __restrict__ ch, const double4* __restrict__ chd, int t0,
-    int t1, double (&out)[K][4]) {
+    int t1, double (&out)[K][kAcc]) {
   constexpr bool hasE = WHAT != kWhatPhi;
   constexpr bool hasP = WHAT == kWhatPhi || WHAT == kWhatESoftPhi;
   constexpr bool soft = WHAT == kWhatESoft || WHAT == kWhatESoftPhi;
@@ -96,6 +110,7 @@ __device__ __forceinline__ void accumulate(
   constexpr bool full64 = MODE == kModeF64;
   constexpr bool naive = MODE == kModeNaive;
   constexpr bool kahan = MODE == kModeKahan;
+  constexpr bool track = MODE == kModeAuto && e32;  // magnitude sum for the check
   constexpr int kGroup = (need64 || kahan) ? 4 : 8;
 
   const float rclamp = __builtin_amdgcn_rsqf(1e-6f);
@@ -107,7 +122,7 @@ __device__ __forceinline__ void accumulate(
     pyd[k] = (double)py[SHARED_XY ? 0 : k];
     pzd[k] = (double)pz[k];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) out[k][c] = 0.0;
+    for (int c = 0; c < kAcc; ++c) out[k][c] = 0.0;
   }
   float na[K][4];  // naive: FP32 sums over the whole charge range
 #pragma unroll
@@ -117,10 +132,13 @@ __device__ __forceinline__ void accumulate(
 
   for (int t = t0; t < t1; ++t) {
     float ta[K][4], tc[K][4];  // per-tile FP32 sums (+ Kahan compensation)
+    float tm[K];               // per-tile sum |q| / r^2 (track only)
 #pragma unroll
-    for (int k = 0; k < K; ++k)
+    for (int k = 0; k < K; ++k) {
+      tm[k] = 0.f;
 #pragma unroll
       for (int c = 0; c < 4; ++c) ta[k][c] = tc[k][c] = 0.f;
+    }
     const int jb = t * kTile;
     // Charges are fetched kGroup at a time (the scalar loads of a group are
     // issued together), so the scalar-cache latency is paid once per group.
@@ -168,7 +186,9 @@ __device__ __forceinline__ void accumulate(
           }
           if constexpr (e32) {
             const float re = soft ? fminf(ri, rclamp) : ri;
-            const float s = (c.w * re) * (re * re);
+            const float re2 = re * re;
+            const float s = (c.w * re) * re2;
+            if constexpr (track) tm[k] = fmaf(fabsf(c.w), re2, tm[k]);
             if constexpr (kahan) {
               kahan_add(s * dx, acc[0], tc[k][0]);
               kahan_add(s * dy, acc[1], tc[k][1]);
@@ -213,6 +233,7 @@ __device__ __forceinline__ void accumulate(
         }
         if constexpr (p32)
           out[k][3] += kahan ? (double)ta[k][3] - (double)tc[k][3] : (double)ta[k][3];
+        if constexpr (track) out[k][4] += (double)tm[k];
       }
     }
   }
@@ -227,20 +248,20 @@ __device__ __forceinline__ void accumulate(
 struct FieldOut {
   void* e;         // (M,3) float or double, or null
   void* phi;       // (M) float or double, or null
-  double* part;    // (splits, M, 4) partial sums when splits > 1, else null
+  double* part;    // (splits, M, kAcc) partial sums when splits > 1, else null
   int f64;         // outputs are double
   long long M;
+  int* fix_count;  // auto-mode check: number of listed points, or null
+  long long* fix_list;  // output indices of points to recompute
 };
 
 template <int WHAT>
 __device__ __forceinline__ void store_point(const FieldOut& o, long long m,
-                                            const double (&acc)[4], int split) {
+                                            const double (&acc)[kAcc], int split) {
   if (o.part) {
-    double* p = o.part + ((long long)split * o.M + m) * 4;
-    p[0] = acc[0];
-    p[1] = acc[1];
-    p[2] = acc[2];
-    p[3] = acc[3];
+    double* p = o.part + ((long long)split * o.M + m) * kAcc;
+#pragma unroll
+    for (int c = 0; c < kAcc; ++c) p[c] = acc[c];
     return;
   }
   const double k = kCoulomb;
@@ -263,6 +284,25 @@ __device__ __forceinline__ void store_point(const FieldOut& o, long long m,
     else
       ((float*)o.phi)[m] = (float)(k * acc[3]);
   }
+}
+
+// True when the FP32 error estimate u * A exceeds kAutoTol * |E|.
+__device__ __forceinline__ bool needs_fix(const double (&acc)[kAcc]) {
+  const double a = acc[4] * (kUnit32 / kAutoTol);
+  return a * a > fma(acc[2], acc[2], fma(acc[1], acc[1], acc[0] * acc[0]));
+}
+
+// Appends the output index g of every active lane with flag set to the fix
+// list: one atomic per wave, slots in lane order.
+__device__ __forceinline__ void append_fix(const FieldOut& o, long long g, bool flag) {
+  const unsigned long long mask = __ballot(flag);
+  if (mask == 0ull) return;
+  const int lane = __lane_id();
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(o.fix_count, __popcll(mask));
+  base = __shfl(base, leader);
+  if (flag) o.fix_list[base + __popcll(mask & ((1ull << lane) - 1ull))] = g;
 }
 
 // Grid kernel: lane -> (line (i,j), chunk of K z-points).
@@ -290,12 +330,16 @@ __global__ __launch_bounds__(256) void field_grid_kernel(
   const int split = blockIdx.y;
   const int t0 = split * tiles_per_split;
   const int t1 = min(t0 + tiles_per_split, ntiles);
-  double acc[K][4];
+  double acc[K][kAcc];
   accumulate<K, true, MODE, WHAT>(px, py, pz, ch, chd, t0, t1, acc);
   const long long base = line * nz + z0;
 #pragma unroll
-  for (int k = 0; k < K; ++k)
-    if (z0 + k < nz) store_point<WHAT>(o, base + k, acc[k], split);
+  for (int k = 0; k < K; ++k) {
+    const bool valid = z0 + k < nz;
+    if (valid) store_point<WHAT>(o, base + k, acc[k], split);
+    if constexpr (MODE == kModeAuto && WHAT != kWhatPhi)
+      if (o.fix_count && !o.part) append_fix(o, base + k, valid && needs_fix(acc[k]));
+  }
 }
 
 // Points kernel: lane owns points base + k*blockDim + tid.
@@ -317,12 +361,15 @@ __global__ __launch_bounds__(256) void field_points_kernel(
   const int split = blockIdx.y;
   const int t0 = split * tiles_per_split;
   const int t1 = min(t0 + tiles_per_split, ntiles);
-  double acc[K][4];
+  double acc[K][kAcc];
   accumulate<K, false, MODE, WHAT>(px, py, pz, ch, chd, t0, t1, acc);
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const long long m = base + (long long)k * blockDim.x;
-    if (m < M) store_point<WHAT>(o, m, acc[k], split);
+    const bool valid = m < M;
+    if (valid) store_point<WHAT>(o, m, acc[k], split);
+    if constexpr (MODE == kModeAuto && WHAT != kWhatPhi)
+      if (o.fix_count && !o.part) append_fix(o, m, valid && needs_fix(acc[k]));
   }
 }
 
@@ -333,17 +380,115 @@ __global__ __launch_bounds__(256) void field_reduce_kernel(long long M,
                                                            FieldOut o) {
   const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= M) return;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  double acc[kAcc] = {0.0, 0.0, 0.0, 0.0, 0.0};
   for (int s = 0; s < splits; ++s) {
-    const double* p = o.part + ((long long)s * M + m) * 4;
-    acc[0] += p[0];
-    acc[1] += p[1];
-    acc[2] += p[2];
-    acc[3] += p[3];
+    const double* p = o.part + ((long long)s * M + m) * kAcc;
+#pragma unroll
+    for (int c = 0; c < kAcc; ++c) acc[c] += p[c];
   }
   FieldOut d = o;
   d.part = nullptr;
   store_point<WHAT>(d, m, acc, 0);
+  if (WHAT != kWhatPhi && o.fix_count) append_fix(o, m, needs_fix(acc));
+}
+
+__global__ void fix_reset_kernel(int* count) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *count = 0;
+}
+
+// Recomputes E at the listed points in the "mixed" scheme (FP64
+// displacement and r^2, FP32 rsq seed + one FP64 Newton step, FP64 sums):
+// one wave per point, lanes stride over the charges, then a fixed-order
+// butterfly reduction.  Single frame (F == 0: grid axes or points) or a
+// batch (F > 0: the batch tables; the frame of output index g is the last
+// one whose first output index is <= g).
+struct FixArgs {
+  const int* count;
+  const long long* list;
+  const float4* ch;
+  const float *ax, *ay, *az, *pts;
+  int nx, ny, nz, npad;
+  int F;
+  const int32_t* geo;
+  const int64_t* out_off;
+  const int32_t* ax_off;
+  const float* axes;
+  const int32_t* chg_off;
+  void* e;
+  int f64;
+};
+
+template <bool SOFT>
+__global__ __launch_bounds__(256) void field_fix_kernel(FixArgs a) {
+  const int n = *a.count;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const double rclampd = rsqrt_f64((double)1e-6f);
+  for (long long t = (long long)blockIdx.x * 4 + wave; t < n; t += (long long)gridDim.x * 4) {
+    const long long g = a.list[t];
+    float px, py, pz;
+    int c0 = 0, c1 = a.npad;
+    if (a.F > 0) {
+      int lo = 0, hi = a.F - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.out_off[mid] <= g) lo = mid; else hi = mid - 1;
+      }
+      const long long m = g - a.out_off[lo];
+      const int nx = a.geo[4 * lo], ny = a.geo[4 * lo + 1], nz = a.geo[4 * lo + 2];
+      const long long line = m / nz;
+      const int z = (int)(m - line * nz), i = (int)(line / ny), j = (int)(line - (long long)i * ny);
+      const float* ax = a.axes + a.ax_off[lo];
+      px = ax[i];
+      py = ax[nx + j];
+      pz = ax[nx + ny + z];
+      c0 = a.chg_off[lo];
+      c1 = a.chg_off[lo + 1];
+    } else if (a.pts) {
+      px = a.pts[3 * g];
+      py = a.pts[3 * g + 1];
+      pz = a.pts[3 * g + 2];
+    } else {
+      const long long line = g / a.nz;
+      const int z = (int)(g - line * a.nz), i = (int)(line / a.ny),
+                j = (int)(line - (long long)i * a.ny);
+      px = a.ax[i];
+      py = a.ay[j];
+      pz = a.az[z];
+    }
+    const double pxd = px, pyd = py, pzd = pz;
+    double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+    for (int jq = c0 + lane; jq < c1; jq += 64) {
+      const float4 c = a.ch[jq];
+      const double dx = pxd - (double)c.x, dy = pyd - (double)c.y, dz = pzd - (double)c.z;
+      const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+      const double y = rsqrt_refine(__builtin_amdgcn_rsqf((float)r2), r2);
+      const double re = SOFT ? fmin(y, rclampd) : y;
+      const double s = ((double)c.w * re) * (re * re);
+      e0 = fma(s, dx, e0);
+      e1 = fma(s, dy, e1);
+      e2 = fma(s, dz, e2);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      e0 += __shfl_xor(e0, off);
+      e1 += __shfl_xor(e1, off);
+      e2 += __shfl_xor(e2, off);
+    }
+    if (lane == 0) {
+      const double k = kCoulomb;
+      if (a.f64) {
+        double* e = (double*)a.e + 3 * g;
+        e[0] = k * e0;
+        e[1] = k * e1;
+        e[2] = k * e2;
+      } else {
+        float* e = (float*)a.e + 3 * g;
+        e[0] = (float)(k * e0);
+        e[1] = (float)(k * e1);
+        e[2] = (float)(k * e2);
+      }
+    }
+  }
 }
 
 // Many grids in one launch: blk[2b] = frame, blk[2b+1] = block within the
@@ -358,6 +503,8 @@ struct BatchFieldArgs {
   void* e;
   void* phi;
   int f64;
+  int* fix_count;
+  long long* fix_list;
 };
 
 template <int K, int MODE, int WHAT>
@@ -385,7 +532,7 @@ __global__ __launch_bounds__(256) void field_batch_kernel(const float4* __restri
   }
   const int c0 = a.chg_off[fr];
   const int ntiles = (a.chg_off[fr + 1] - c0) / kTile;
-  double acc[K][4];
+  double acc[K][kAcc];
   accumulate<K, true, MODE, WHAT>(px, py, pz, ch + c0, chd ? chd + c0 : nullptr, 0, ntiles, acc);
   FieldOut o;
   const size_t es = a.f64 ? sizeof(double) : sizeof(float);
@@ -395,10 +542,16 @@ __global__ __launch_bounds__(256) void field_batch_kernel(const float4* __restri
   o.part = nullptr;
   o.f64 = a.f64;
   o.M = (long long)nx * ny * nz;
+  o.fix_count = a.fix_count;
+  o.fix_list = a.fix_list;
   const long long base = line * nz + z0;
 #pragma unroll
-  for (int k = 0; k < K; ++k)
-    if (z0 + k < nz) store_point<WHAT>(o, base + k, acc[k], 0);
+  for (int k = 0; k < K; ++k) {
+    const bool valid = z0 + k < nz;
+    if (valid) store_point<WHAT>(o, base + k, acc[k], 0);
+    if constexpr (MODE == kModeAuto && WHAT != kWhatPhi)
+      if (a.fix_count) append_fix(o, off + base + k, valid && needs_fix(acc[k]));
+  }
 }
 
 template <int K, int MODE, int WHAT>
@@ -507,6 +660,28 @@ int pick_splits(long long lanes, int ntiles, int num_cus, int forced) {
   return (int)std::max<long long>(1, s);
 }
 
+// Reserves the auto-mode check's counter (offset 0) and list (offset 256)
+// for up to M points and queues the counter reset.
+int fix_prepare(cpet_ctx* ctx, long long M, int** count, long long** list) {
+  void* buf = nullptr;
+  const int rc = ctx_buf(ctx, kSlotFix, 256 + (size_t)M * sizeof(long long), &buf);
+  if (rc) return rc;
+  *count = (int*)buf;
+  *list = (long long*)((char*)buf + 256);
+  hipLaunchKernelGGL(fix_reset_kernel, dim3(1), dim3(64), 0, ctx->stream, *count);
+  ctx->fix_count = *count;
+  return CPET_OK;
+}
+
+void fix_launch(cpet_ctx* ctx, long long M, bool soft, const FixArgs& a) {
+  const long long want = (M + 3) / 4;
+  const int blocks = (int)std::max<long long>(1, std::min<long long>(want, (long long)ctx->num_cus * 8));
+  if (soft)
+    hipLaunchKernelGGL(field_fix_kernel<true>, dim3(blocks), dim3(256), 0, ctx->stream, a);
+  else
+    hipLaunchKernelGGL(field_fix_kernel<false>, dim3(blocks), dim3(256), 0, ctx->stream, a);
+}
+
 }  // namespace
 
 bool field_mode_valid(int mode) { return mode >= kModeAuto && mode <= kModeMixed; }
@@ -534,6 +709,9 @@ int field_launch(cpet_ctx* ctx, bool grid_mode, int nx, int ny, int nz,
   o.part = nullptr;
   o.f64 = (flags & CPET_F_OUT_F64) ? 1 : 0;
   o.M = M;
+  o.fix_count = nullptr;
+  o.fix_list = nullptr;
+  ctx->fix_count = nullptr;
 
   int K;
   long long lanes;
@@ -561,10 +739,15 @@ int field_launch(cpet_ctx* ctx, bool grid_mode, int nx, int ny, int nz,
   const int tps = (ntiles + splits - 1) / splits;
   if (splits > 1) {
     void* part = nullptr;
-    int rc = ctx_buf(ctx, kSlotPartial, (size_t)splits * M * 4 * sizeof(double),
+    int rc = ctx_buf(ctx, kSlotPartial, (size_t)splits * M * kAcc * sizeof(double),
                      &part);
     if (rc) return rc;
     o.part = (double*)part;
+  }
+  const bool check = mode == kModeAuto && E_out != nullptr;
+  if (check) {
+    const int rc = fix_prepare(ctx, M, &o.fix_count, &o.fix_list);
+    if (rc) return rc;
   }
   const long long blocks_x = grid_mode ? (lanes + 255) / 256
                                        : (M + 256LL * K - 1) / (256LL * K);
@@ -612,6 +795,26 @@ int field_launch(cpet_ctx* ctx, bool grid_mode, int nx, int ny, int nz,
     CPET_HIP_TRY(hipGetLastError());
   }
   if ((prc = prof_mark(ctx, 2))) return prc;
+  if (check) {
+    FixArgs a = {};
+    a.count = o.fix_count;
+    a.list = o.fix_list;
+    a.ch = ch;
+    a.ax = grid_mode ? ax : nullptr;
+    a.ay = ay;
+    a.az = az;
+    a.pts = grid_mode ? nullptr : pts;
+    a.nx = nx;
+    a.ny = ny;
+    a.nz = nz;
+    a.npad = npad;
+    a.F = 0;
+    a.e = E_out;
+    a.f64 = o.f64;
+    fix_launch(ctx, M, what != kWhatEHard, a);
+    CPET_HIP_TRY(hipGetLastError());
+  }
+  if ((prc = prof_mark(ctx, 3))) return prc;
   return CPET_OK;
 }
 
@@ -680,6 +883,11 @@ int field_batch_launch(cpet_ctx* ctx, int F, const int32_t* geo3, const float* a
   a.e = E_out;
   a.phi = phi_out;
   a.f64 = (flags & CPET_F_OUT_F64) ? 1 : 0;
+  a.fix_count = nullptr;
+  a.fix_list = nullptr;
+  ctx->fix_count = nullptr;
+  const bool check = mode == kModeAuto && E_out != nullptr;
+  if (check && (rc = fix_prepare(ctx, pts, &a.fix_count, &a.fix_list))) return rc;
   const int nblk = (int)(blk.size() / 2);
   if ((rc = prof_mark(ctx, 0))) return rc;
   bool ok = false;
@@ -695,6 +903,23 @@ int field_batch_launch(cpet_ctx* ctx, int F, const int32_t* geo3, const float* a
   }
   CPET_HIP_TRY(hipGetLastError());
   if ((rc = prof_mark(ctx, 1))) return rc;
+  if (check) {
+    FixArgs fa = {};
+    fa.count = a.fix_count;
+    fa.list = a.fix_list;
+    fa.ch = ch;
+    fa.F = F;
+    fa.geo = a.geo;
+    fa.out_off = a.out_off;
+    fa.ax_off = a.ax_off;
+    fa.axes = axes_dev;
+    fa.chg_off = d_chg_off;
+    fa.e = E_out;
+    fa.f64 = a.f64;
+    fix_launch(ctx, pts, what != kWhatEHard, fa);
+    CPET_HIP_TRY(hipGetLastError());
+  }
+  if ((rc = prof_mark(ctx, 2))) return rc;
   return CPET_OK;
 }
 

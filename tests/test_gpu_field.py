@@ -186,3 +186,49 @@ def test_device_tensor_path(gpu_ctx):
     torch.cuda.synchronize()
     np.testing.assert_array_equal(Et.cpu().numpy(), En)
     np.testing.assert_array_equal(pt.cpu().numpy(), pn)
+
+
+def _cancelling_frame():
+    """771 charges at >= 1.5 A from a +-0.6 A box holding a 2,535-point grid:
+    at some points the field cancels to a few % of its RMS, where FP32 pair
+    rounding alone (even summed exactly) exceeds 1e-5 relative."""
+    from pycpet_amd import workloads as W
+
+    x, q = W.ball_exclusion(771, 12.0, 0.6, 1.5, seed=100)
+    ax = np.linspace(-0.5, 0.5, 13).astype(np.float32)
+    ay = np.linspace(-0.3, 0.4, 15).astype(np.float32)
+    return x, q, (ax, ay, ax)
+
+
+def test_auto_recomputes_cancelling_points(gpu_ctx, oracle_mod):
+    """auto = FP32 tiles plus an FP64 recompute of the points whose error
+    estimate u * sum|q|/r^2 exceeds 4e-6 |E|: the bound holds pointwise even
+    where the plain FP32 tiling (prec="tiled") is allowed to miss it, and the
+    recomputed points equal the all-"mixed" result to FP64 rounding."""
+    from pycpet_amd import ops
+
+    x, q, axes = _cancelling_frame()
+    pts = mesh(axes)
+    E64, _ = oracle_mod.field_f64(pts, x, q, want_phi=True)
+    for grid in (True, False):
+        run = (lambda **kw: ops.field_grid(*axes, x, q, out_f64=True, ctx=gpu_ctx, **kw)[0]) \
+            if grid else (lambda **kw: ops.field_points(pts, x, q, out_f64=True, ctx=gpu_ctx,
+                                                        **kw)[0])
+        E = run()
+        n_fixed = gpu_ctx.refined_points()
+        Et = run(prec="tiled")
+        assert gpu_ctx.refined_points() == 0
+        Em = run(prec="mixed")
+        err_auto, err_tiled = rel_err(E, E64, 0.0)[0], rel_err(Et, E64, 0.0)[0]
+        print(f"grid={grid} refined {n_fixed}/{len(pts)} err auto {err_auto:.2e} "
+              f"tiled {err_tiled:.2e}", flush=True)
+        assert 0 < n_fixed < len(pts)
+        assert err_auto <= TOL
+        fixed = np.any(E != Et, axis=1)
+        assert fixed.sum() <= n_fixed
+        assert rel_err(E[fixed], Em[fixed], 0.0)[0] < 1e-11  # summation order only
+        # E together with phi (volume semantics) takes the same check
+        E2, phi2 = ops.field_grid(*axes, x, q, want_phi=True, out_f64=True, ctx=gpu_ctx) \
+            if grid else ops.field_points(pts, x, q, want_phi=True, out_f64=True, ctx=gpu_ctx)
+        assert gpu_ctx.refined_points() == n_fixed
+        np.testing.assert_array_equal(E2, E)
