@@ -12,7 +12,9 @@ import os
 import threading
 from pathlib import Path
 
-LIB_PATH = Path(__file__).resolve().with_name("libcpet_hip.so")
+# CPET_HIP_LIB selects another build of the library (for A/B timing).
+LIB_PATH = Path(os.environ.get("CPET_HIP_LIB") or
+                Path(__file__).resolve().with_name("libcpet_hip.so"))
 
 # status codes (include/cpet_hip.h)
 CPET_OK = 0

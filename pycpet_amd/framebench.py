@@ -13,7 +13,9 @@ per frame), inputs resident in HBM, timed three ways:
   batched   all volume frames in one launch and all streamline frames in one
             persistent launch, on two streams.
 serve: Poisson arrivals (pycpet_amd.serve.poisson_requests) served with
-continuous batching vs one request at a time; p50/p99 latency and frames/s.
+continuous batching vs one request at a time (stopped after --plain-max-s,
+unserved requests counted with their latency lower bound); p50/p99 latency
+and frames/s.
 Prints one JSON object.
 """
 from __future__ import annotations
@@ -109,7 +111,8 @@ def run_batch(n_frames: int, topo_starts: int, seed: int = 3) -> dict:
     return out
 
 
-def run_serve(rate: float, duration: float, lanes: int, seed: int = 0) -> dict:
+def run_serve(rate: float, duration: float, lanes: int, seed: int = 0,
+              plain_max_s: float = 120.0) -> dict:
     import torch
 
     from . import serve
@@ -123,7 +126,8 @@ def run_serve(rate: float, duration: float, lanes: int, seed: int = 0) -> dict:
     for r in reqs:
         r.result = None
     torch.cuda.synchronize()
-    pl = serve.serve_plain(reqs).summary()
+    pl = serve.serve_plain(reqs, max_wall_s=plain_max_s,
+                           progress=lambda m: print(m, file=sys.stderr, flush=True)).summary()
     print(f"plain: {pl}", file=sys.stderr, flush=True)
     kinds = {k: sum(r.kind == k for r in reqs) for k in ("volume", "volume_ESP", "topo")}
     return {"workload": "poisson_serving", "rate_req_per_s": rate, "duration_s": duration,
@@ -144,11 +148,13 @@ def main(argv=None) -> int:
     s.add_argument("--rate", type=float, default=200.0)
     s.add_argument("--duration", type=float, default=10.0)
     s.add_argument("--lanes", type=int, default=4)
+    s.add_argument("--plain-max-s", type=float, default=120.0,
+                   help="stop the one-at-a-time baseline after this wall time")
     a = ap.parse_args(argv)
     if a.cmd == "batch":
         out = run_batch(a.frames, a.topo_starts)
     else:
-        out = run_serve(a.rate, a.duration, a.lanes)
+        out = run_serve(a.rate, a.duration, a.lanes, plain_max_s=a.plain_max_s)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -208,8 +208,14 @@ class Server:
 
 
 def serve_plain(requests: List[Request], device: int = 0,
-                stats: Optional[ServeStats] = None) -> ServeStats:
-    """One request at a time, per-frame entry points, one stream."""
+                stats: Optional[ServeStats] = None, max_wall_s: Optional[float] = None,
+                progress=None) -> ServeStats:
+    """One request at a time, per-frame entry points, one stream.
+
+    With max_wall_s, serving stops at that wall time; requests not served
+    by then are counted in `stats.rejected` and their latency enters the
+    percentiles as the lower bound (stop time - arrival).  `progress`, if
+    given, is called with a status string every 100 requests."""
     import torch
 
     stats = stats or ServeStats()
@@ -217,10 +223,17 @@ def serve_plain(requests: List[Request], device: int = 0,
     ctx = nat.Context(device)
     T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
     queue = collections.deque(sorted(requests, key=lambda r: r.arrival))
+    busy = collections.Counter()
     t0 = time.perf_counter()
     while queue:
-        r = queue.popleft()
         now = time.perf_counter() - t0
+        if max_wall_s is not None and now > max_wall_s:
+            for r in queue:
+                stats.latencies.append(max(now - r.arrival, 0.0))
+                stats.rejected += 1
+            queue.clear()
+            break
+        r = queue.popleft()
         if r.arrival > now:
             time.sleep(r.arrival - now)
         r.t_admit = time.perf_counter() - t0
@@ -236,8 +249,12 @@ def serve_plain(requests: List[Request], device: int = 0,
             out = res.feats
         r.result = out.cpu().numpy()
         r.t_done = time.perf_counter() - t0
+        busy[r.kind] += r.t_done - r.t_admit
         stats.latencies.append(r.t_done - r.arrival)
         stats.completed += 1
+        if progress is not None and stats.completed % 100 == 0:
+            progress(f"plain: {stats.completed} done at {r.t_done:.1f} s, service s by kind "
+                     f"{dict((k, round(v, 2)) for k, v in busy.items())}")
     stats.wall = time.perf_counter() - t0
     return stats
 
