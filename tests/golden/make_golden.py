@@ -5,9 +5,13 @@ them (CPET/utils/c_ops.py:250-341 and CPET/utils/calculator.py:430-475).
 
 Run in a checkout where the reference source exists:
     make -C oracle ref && python tests/golden/make_golden.py
-    python tests/golden/make_golden.py pdb      # only ref_pdb_rows.pdb
+    python tests/golden/make_golden.py pdb      # only the text fixtures
 
 Each .npz holds the seeded inputs and the reference's FP32 outputs only.
+examples_point_field/ holds the reference's point-field example: its two
+frames (examples/1A_point-field/pdb, gzipped), and in examples.json the
+options files of examples 1A, 1B and 5A with the point_field.dat each wrote
+(examples/*/outdir/point_field.dat).
 ref_pdb_rows.pdb holds lines of the reference's own test frame
 (tests/test_files/test_large.pdb): its header, first records, a TER line
 and the last HETATM records (5-digit serials, 4-digit residue numbers).
@@ -56,6 +60,30 @@ def topo(ref):
                         n_iter=c.n_iter, h=np.float32(c.h), dims=c.dims, ret=ret)
 
 
+def point_field_examples():
+    import gzip
+    import json
+    import re
+
+    ex = Path("/root/reference/examples")
+    out = OUT / "examples_point_field"
+    out.mkdir(exist_ok=True)
+    for f in sorted((ex / "1A_point-field" / "pdb").glob("*.pdb")):
+        # mtime=0 keeps the archive bytes reproducible
+        with open(out / (f.name + ".gz"), "wb") as fh, \
+                gzip.GzipFile(filename=f.name, mode="wb", fileobj=fh, mtime=0) as gz:
+            gz.write(f.read_bytes())
+    cases = {}
+    for d in ("1A_point-field", "1B_point-field-oriented", "5A_filter-ids"):
+        opts = json.loads((ex / d / "options" / "options.json").read_text())
+        expected = {}
+        for line in (ex / d / "outdir" / "point_field.dat").read_text().splitlines():
+            name, vec = line.split(":", 1)
+            expected[name] = [float(v) for v in re.findall(r"[-+0-9.eE]+", vec)]
+        cases[d] = {"options": opts, "expected": expected}
+    (out / "examples.json").write_text(json.dumps(cases, indent=1) + "\n")
+
+
 def pdb_rows():
     src = Path("/root/reference/tests/test_files/test_large.pdb").read_text().splitlines(True)
     ter = next(i for i, ln in enumerate(src) if ln.startswith("TER"))
@@ -65,7 +93,8 @@ def pdb_rows():
 
 if __name__ == "__main__":
     pdb_rows()
-    if sys.argv[1:] == ["pdb"]:  # text fixture only
+    point_field_examples()
+    if sys.argv[1:] == ["pdb"]:  # text fixtures only
         sys.exit(0)
     r = Ref()
     volume(r)
