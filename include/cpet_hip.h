@@ -248,6 +248,32 @@ int cpet_ctx_refined_points(cpet_ctx *ctx, long long *n);
  *   workgroups), kind 4 = batch volume (a = points per lane). */
 int cpet_set_tuning(cpet_ctx *ctx, int kind, int a, int b);
 
+/* ------------------------------------------------------ cluster step
+ * The step after "topo" (CPET/source/cluster.py:126-130): per-frame
+ * histograms of the streamline features and their chi-square distances. */
+
+/* Per-frame normalised 2-D histograms, as make_histograms
+ * (CPET/utils/calculator.py:596-718) builds them: for frame f (feature rows
+ * offsets[f] .. offsets[f+1]-1 of feats (P,2) = [distance, curvature]),
+ * counts over the shared edges with numpy.histogramdd's rule (searchsorted
+ * right; a value equal to the last edge in the last bin; others outside are
+ * dropped), then divided by the frame's own in-range count (0/0 = NaN, as
+ * numpy).  hists (F, nb_d*nb_c) double, row-major [bd][bc].  offsets (F+1,
+ * int64, starting at 0) and the edges (nb_d+1, nb_c+1 doubles, increasing)
+ * are host arrays; feats and hists are device pointers with CPET_F_DEVICE. */
+int cpet_cluster_histograms(cpet_ctx *ctx, int F, const int64_t *offsets, const double *feats,
+                            int nb_d, const double *edges_d, int nb_c, const double *edges_c,
+                            double *hists, uint32_t flags);
+
+/* out (F,F) double: out[i][j] = sum over bins with h_i + h_j != 0 of
+ * (h_i - h_j)^2 / (h_i + h_j), halved; zero diagonal.  Replaces
+ * distance_numpy + construct_distance_matrix (CPET/utils/calculator.py:
+ * 975-978, 1000-1015; sklearn pairwise_distances over that metric).  FP64;
+ * bins are summed in order (numpy sums pairwise: agreement to FP64
+ * rounding). */
+int cpet_chi2_matrix(cpet_ctx *ctx, int F, int B, const double *hists, double *out,
+                     uint32_t flags);
+
 /* ------------------------------------------------------ frame text I/O
  * Host-only (no GPU needed).  These feed frames to the kernels and write
  * their results, at the rate a serving loop needs. */
@@ -277,6 +303,15 @@ int cpet_pdb_parse(const char *path, int n_atoms, int get_charges, double *xyz, 
  * Rows are formatted by n_threads threads (0 = all cores). */
 int cpet_write_rows(const char *path, const char *header, const void *data, int dtype,
                     int64_t rows, int cols, int fmt, int n_threads);
+
+/* Reads the numeric rows of a text file as make_histograms reads a .top
+ * file (CPET/utils/calculator.py:603-633): lines starting with '#' are
+ * skipped; every other line gives its first `cols` whitespace-separated
+ * values, parsed as Python's float() does (correctly rounded).  With out ==
+ * NULL only counts the rows (*n_rows); otherwise fills out (cap rows x cols)
+ * and sets *n_rows.  A line with fewer than cols values, or an unparsable
+ * value, is an error naming the line. */
+int cpet_read_rows(const char *path, int cols, double *out, int64_t cap, int64_t *n_rows);
 
 #ifdef __cplusplus
 }

@@ -15,6 +15,10 @@ Contents
                             build time; the built .so may travel with the tree).
   grid_axes, histogram2d, filter_* , box_basis
                             numpy restatements of the host-side pieces.
+  cluster_histograms, chi2_matrix
+                            the clustering step exactly as the reference runs
+                            it: numpy.histogram2d + scipy.stats.iqr, and
+                            sklearn pairwise_distances over distance_numpy.
 
 Parity pins (where each function is checked) are listed per function; a
 function with no pin would say "parity unpinned" here.
@@ -292,3 +296,49 @@ def filter_atomspec_drop(ID, filter_dict, intersect):
                 if ok:
                     drop.add(j)
     return sorted(drop)
+
+
+# ----------------------------------------------------------- cluster step
+def cluster_histograms(features):
+    """make_histograms (CPET/utils/calculator.py:596-718) on in-memory
+    per-frame (L, 2) features: the reference's own steps with the libraries
+    it calls (scipy.stats.iqr, numpy.histogram2d).  Returns (hists (F, B),
+    (xedges, yedges)).  Pinned by tests/test_oracle_cluster.py (brute-force
+    counts, normalisation, edge placement)."""
+    from scipy.stats import iqr
+
+    lens = np.array([len(f) for f in features], dtype=int)
+    n = lens[0] if np.all(lens == lens[0]) else np.mean(lens)
+    d = np.concatenate([np.asarray(f, float)[:, 0] for f in features])
+    c = np.concatenate([np.asarray(f, float)[:, 1] for f in features])
+    dres = 2 * iqr(d) / (n ** (1 / 3))
+    cres = 2 * iqr(c) / (n ** (1 / 3))
+    nd = int((np.max(d) - np.min(d)) / dres)
+    nc = int((np.max(c) - np.min(c)) / cres)
+    out, edges = [], None
+    for f in features:
+        f = np.asarray(f, float)
+        a, xe, ye = np.histogram2d(f[:, 0], f[:, 1], bins=[nd, nc],
+                                   range=[[np.min(d), np.max(d)], [np.min(c), np.max(c)]])
+        out.append((a / np.sum(a)).flatten())
+        edges = (xe, ye)
+    return np.array(out), edges
+
+
+def chi2_distance(h1, h2):
+    """distance_numpy, CPET/utils/calculator.py:975-978."""
+    a = (h1 - h2) ** 2
+    b = h1 + h2
+    return np.sum(np.divide(a, b, out=np.zeros_like(a), where=b != 0)) / 2.0
+
+
+def chi2_matrix(hists):
+    """construct_distance_matrix (CPET/utils/calculator.py:1000-1015):
+    sklearn pairwise_distances with distance_numpy, zero diagonal.  Pinned
+    by tests/test_oracle_cluster.py (closed forms: identical -> 0, disjoint
+    supports -> 1, a hand-computed pair, symmetry)."""
+    from sklearn.metrics.pairwise import pairwise_distances
+
+    m = pairwise_distances(np.asarray(hists, float), metric=chi2_distance, n_jobs=1)
+    np.fill_diagonal(m, 0.0)
+    return m

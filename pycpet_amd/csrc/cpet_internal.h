@@ -108,6 +108,9 @@ enum Slot : int {
   kSlotBatchTables,  // batch offset tables
   kSlotBatchWork,    // batch work lists / counters
   kSlotFix,          // auto-mode check: counter + list of points to recompute
+  kSlotCluster0,     // cluster step: staged features / histograms
+  kSlotCluster1,     // cluster step: edges, offsets, counts
+  kSlotCluster2,     // cluster step: output
   kSlotCount
 };
 
@@ -143,5 +146,22 @@ int pack_charges_batch(cpet_ctx* ctx, int F, const int32_t* nch, const float* x,
 // intervals between consecutive marks in ctx->phase_ms.
 int prof_mark(cpet_ctx* ctx, int i);
 int prof_collect(cpet_ctx* ctx, int n);
+
+// numpy.searchsorted(edges, v, side='right') with the histogramdd rule that
+// a value equal to the last edge falls into the last bin.  Returns the bin
+// or -1 for an outlier (including NaN).
+__device__ __forceinline__ int hist_bin(const double* __restrict__ e, int nb,
+                                        double v) {
+  int lo = 0, hi = nb + 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (e[mid] <= v)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  if (v == e[nb]) lo -= 1;
+  return (lo >= 1 && lo <= nb) ? lo - 1 : -1;
+}
 
 }  // namespace cpet

@@ -87,7 +87,7 @@ Field field(const char* l, size_t n, size_t a, size_t b) {
   return {l + s, e - s};
 }
 
-// Fixed-width numeric fields are at most 9 characters wide.
+// Copies a field into a NUL-terminated buffer; false if empty or too long.
 bool terminated(Field f, char* buf, size_t cap) {
   if (f.n == 0 || f.n >= cap) return false;
   std::memcpy(buf, f.p, f.n);
@@ -109,7 +109,7 @@ bool parse_int(Field f, int32_t* v) {
 // Python's float() accepts decimal literals, inf/infinity and nan; strtod also
 // takes hexadecimal ("0x1p3"), which float() rejects.
 bool parse_float(Field f, double* v) {
-  char buf[16];
+  char buf[64];  // "%.18e" values are 24-25 characters
   if (!terminated(f, buf, sizeof(buf)) || std::strpbrk(buf, "xX")) return false;
   char* end = nullptr;
   const double x = std::strtod(buf, &end);  // correctly rounded (glibc)
@@ -302,6 +302,46 @@ int cpet_write_rows(const char* path, const char* header, const void* data, int 
     return CPET_ERR_ARG;
   }
   return CPET_OK;
+}
+
+int cpet_read_rows(const char* path, int cols, double* out, int64_t cap, int64_t* n_rows) {
+  if (!path || cols <= 0 || cap < 0 || !n_rows) {
+    set_error("cpet_read_rows: bad arguments");
+    return CPET_ERR_ARG;
+  }
+  std::string s;
+  if (!read_file(path, s)) {
+    set_error("cannot read %s", path);
+    return CPET_ERR_ARG;
+  }
+  int64_t n = 0;
+  int lineno = 0, rc = CPET_OK;
+  for_lines(s, [&](const char* l, size_t len) {
+    ++lineno;
+    if (rc != CPET_OK || (len > 0 && l[0] == '#')) return;
+    if (out) {
+      if (n >= cap) {
+        set_error("%s has more than %lld rows", path, (long long)cap);
+        rc = CPET_ERR_ARG;
+        return;
+      }
+      size_t k = 0;
+      for (int c = 0; c < cols; ++c) {
+        while (k < len && py_space((unsigned char)l[k])) ++k;
+        size_t e = k;
+        while (e < len && !py_space((unsigned char)l[e])) ++e;
+        if (e == k || !parse_float(Field{l + k, e - k}, &out[n * cols + c])) {
+          set_error("%s line %d: expected %d numbers", path, lineno, cols);
+          rc = CPET_ERR_ARG;
+          return;
+        }
+        k = e;
+      }
+    }
+    ++n;
+  });
+  *n_rows = n;
+  return rc;
 }
 
 }  // extern "C"

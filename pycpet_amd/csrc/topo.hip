@@ -417,23 +417,6 @@ __device__ __forceinline__ double curve_f64(const double (&a)[3],
   return nc / (na * na * na);
 }
 
-// numpy.searchsorted(edges, v, side='right') with the histogramdd rule that
-// a value equal to the last edge falls into the last bin.  Returns the bin
-// or -1 for an outlier (including NaN).
-__device__ __forceinline__ int hist_bin(const double* __restrict__ e, int nb,
-                                        double v) {
-  int lo = 0, hi = nb + 1;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (e[mid] <= v)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  if (v == e[nb]) lo -= 1;
-  return (lo >= 1 && lo <= nb) ? lo - 1 : -1;
-}
-
 // Features (and histogram count) of streamline l of one frame.
 __device__ __forceinline__ void epilogue_one(const float4* __restrict__ ch, int npad,
                                              const float* starts, const double* fin,
