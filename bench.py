@@ -99,7 +99,8 @@ def main() -> int:
     dev = torch.device("cuda:0")
     case = W.topo_5k()
     T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
-    starts, n_iter, dims, x, q = T(case.starts), T(case.n_iter), T(case.dims), T(case.x), T(case.q)
+    starts, n_iter, x, q = T(case.starts), T(case.n_iter), T(case.x), T(case.q)
+    dims = case.dims  # a host parameter, like h
     ctx = nat.Context(0)
     ctx.set_profiling(True)
 

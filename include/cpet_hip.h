@@ -144,7 +144,11 @@ int cpet_field_grid(cpet_ctx *ctx, int nx, int ny, int nz, const float *ax,
  *          given edges (nb_d+1 and nb_c+1 doubles, increasing), as
  *          np.histogram2d does in CPET/utils/calculator.py:702-707.
  * Positions are integrated in FP64 with FP32 tiled field sums; the four
- * curvature-defining evaluations run in FP64. */
+ * curvature-defining evaluations run in FP64.
+ * dims (like h) is a host array in every mode; with CPET_F_DEVICE the other
+ * arrays, the edges included, are device pointers, and with CPET_F_NO_SYNC
+ * as well the call neither waits on nor reads back from the stream (so it
+ * can be captured into a HIP graph once its scratch has grown). */
 int cpet_topo(cpet_ctx *ctx, int N, int L, const float *starts,
               const int32_t *n_iter, float h, const float *dims,
               const float *x, const float *q, double *feats, int32_t *steps,

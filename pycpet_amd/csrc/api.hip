@@ -536,18 +536,10 @@ int cpet_topo(cpet_ctx* ctx, int N, int L, const float* starts,
     return CPET_ERR_ARG;
   }
   const bool dev = (flags & CPET_F_DEVICE) != 0;
+  // dims is a parameter like h: a host array in every mode, so a
+  // device-pointer call never waits on the stream (and can be captured)
   float hdims[3];
-  if (dev) {
-    // dims is tiny; read it back so the kernel gets plain values
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    int rc = ctx_activate(ctx);
-    if (rc) return rc;
-    CPET_HIP_TRY(hipMemcpyAsync(hdims, dims, sizeof(hdims), hipMemcpyDeviceToHost,
-                                ctx->stream));
-    CPET_HIP_TRY(hipStreamSynchronize(ctx->stream));
-  } else {
-    std::memcpy(hdims, dims, sizeof(hdims));
-  }
+  std::memcpy(hdims, dims, sizeof(hdims));
   if (!finite3(hdims) || !(hdims[0] > 0.f && hdims[1] > 0.f && hdims[2] > 0.f)) {
     set_error("box half-widths must be finite and > 0");
     return CPET_ERR_ARG;

@@ -36,8 +36,7 @@ def _frames_to_device(frames, torch, dev):
         if f["kind"] == "volume":
             g["axes"] = tuple(T(a) for a in f["axes"])
         else:
-            g.update(starts=T(f["starts"]), n_iter=T(f["n_iter"]), h=f["h"], dims=f["dims"],
-                     dims_t=T(f["dims"]))
+            g.update(starts=T(f["starts"]), n_iter=T(f["n_iter"]), h=f["h"], dims=f["dims"])
         out.append(g)
     return out
 
@@ -63,7 +62,7 @@ def run_batch(n_frames: int, topo_starts: int, seed: int = 3) -> dict:
             else:
                 if plain:
                     ctx.set_tuning(2, 1, (f["starts"].shape[0] + 63) // 64)
-                ops.topo(f["starts"], f["n_iter"], f["h"], f["dims_t"], f["x"], f["q"], ctx=ctx)
+                ops.topo(f["starts"], f["n_iter"], f["h"], f["dims"], f["x"], f["q"], ctx=ctx)
             torch.cuda.synchronize()
         ctx.set_tuning(0, 0, 0)
         ctx.set_tuning(2, 0, 0)

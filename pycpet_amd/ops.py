@@ -142,15 +142,21 @@ def topo(starts, n_iter, h: float, dims, x, q, *, edges=None,
 
     edges: optional (edges_d, edges_c) increasing float64 bin edges; the
     histogram counts the float32-rounded features with numpy.histogramdd
-    rules.
+    rules.  dims is read on the host in every mode (pass a numpy array, or
+    a tensor, which is copied to the host).
     """
     ctx = ctx or nat.default_context()
     flags = 0
     h = float(np.float32(h))
+    if _is_torch(dims):
+        dims = dims.detach().cpu().numpy()
+    dims = _host_f32(dims).ravel()
+    if dims.size != 3:
+        raise ValueError("dims must hold 3 half-widths")
     if _is_torch(x):
         import torch
 
-        dev = _torch_dev_args(ctx, starts, n_iter, dims, x, q)
+        dev = _torch_dev_args(ctx, starts, n_iter, x, q)
         L = starts.shape[0]
         feats = torch.empty((L, 2), dtype=torch.float64, device=dev)
         steps = torch.empty((L,), dtype=torch.int32, device=dev)
@@ -166,7 +172,6 @@ def topo(starts, n_iter, h: float, dims, x, q, *, edges=None,
     else:
         starts = _host_f32(starts, 3)
         n_iter = np.ascontiguousarray(n_iter, dtype=np.int32).ravel()
-        dims = _host_f32(dims).ravel()
         x = _host_f32(x, 3)
         q = _host_f32(q).ravel()
         L = starts.shape[0]

@@ -244,7 +244,7 @@ def serve_plain(requests: List[Request], device: int = 0,
                                     ctx=ctx)
             out = E if r.kind == "volume" else phi
         else:
-            res = ops.topo(T(d["starts"]), T(d["n_iter"]), d["h"], T(d["dims"]), T(d["x"]),
+            res = ops.topo(T(d["starts"]), T(d["n_iter"]), d["h"], d["dims"], T(d["x"]),
                            T(d["q"]), ctx=ctx)
             out = res.feats
         r.result = out.cpu().numpy()
