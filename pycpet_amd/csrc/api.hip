@@ -96,6 +96,7 @@ int ctx_buf(cpet_ctx* ctx, int slot, size_t bytes, void** out) {
   DevBuf& b = ctx->bufs[slot];
   if (bytes == 0) bytes = 256;
   if (b.bytes < bytes) {
+    for (auto& t : ctx->tables) t.dst = nullptr;  // cached tables may move
     if (b.ptr) {
       // the old buffer may still be read by queued work
       CPET_HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -114,6 +115,24 @@ int ctx_buf(cpet_ctx* ctx, int slot, size_t bytes, void** out) {
     b.bytes = want;
   }
   *out = b.ptr;
+  return CPET_OK;
+}
+
+int upload_table(cpet_ctx* ctx, int key, void* dst, const void* src, size_t bytes) {
+  auto& t = ctx->tables[key];
+  const char* d = (const char*)dst;
+  if (t.dst == d && t.data.size() == bytes &&
+      (bytes == 0 || std::memcmp(t.data.data(), src, bytes) == 0))
+    return CPET_OK;
+  for (int k = 0; k < kTabCount; ++k) {
+    auto& o = ctx->tables[k];
+    if (k != key && o.dst && o.dst < d + bytes && d < o.dst + o.data.size()) o.dst = nullptr;
+  }
+  t.dst = nullptr;
+  t.data.assign((const char*)src, (const char*)src + bytes);
+  if (bytes)
+    CPET_HIP_TRY(hipMemcpyAsync(dst, t.data.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
+  t.dst = d;
   return CPET_OK;
 }
 
@@ -192,10 +211,10 @@ int pack_charges_batch(cpet_ctx* ctx, int F, const int32_t* nch, const float* x,
   if ((rc = ctx_buf(ctx, kSlotBatchTables, tab_bytes, &tab))) return rc;
   int64_t* d_xoff = (int64_t*)tab;
   int32_t* d_coff = (int32_t*)(d_xoff + F + 1);
-  CPET_HIP_TRY(hipMemcpyAsync(d_xoff, x_off.data(), (F + 1) * sizeof(int64_t),
-                              hipMemcpyHostToDevice, ctx->stream));
-  CPET_HIP_TRY(hipMemcpyAsync(d_coff, chg_off.data(), (F + 1) * sizeof(int32_t),
-                              hipMemcpyHostToDevice, ctx->stream));
+  if ((rc = upload_table(ctx, kTabPackX, d_xoff, x_off.data(), (F + 1) * sizeof(int64_t))))
+    return rc;
+  if ((rc = upload_table(ctx, kTabPackC, d_coff, chg_off.data(), (F + 1) * sizeof(int32_t))))
+    return rc;
   void* d = nullptr;
   if ((rc = ctx_buf(ctx, kSlotCharges, (size_t)total * sizeof(float4), &d))) return rc;
   void* d64 = nullptr;

@@ -83,6 +83,14 @@ struct cpet_ctx {
   // device counter of the points the last auto-mode field call recomputed
   // in FP64 (null when that call had no check)
   int* fix_count = nullptr;
+  // last host table uploaded per key (upload_table): an identical table at
+  // the same device address is not copied again, so repeated calls of one
+  // shape enqueue kernels only (and can be captured into a HIP graph)
+  struct TableCache {
+    const char* dst = nullptr;
+    std::vector<char> data;
+  };
+  TableCache tables[16];
 };
 
 namespace cpet {
@@ -105,8 +113,10 @@ enum Slot : int {
   kSlotFrame0,       // frame-transform scratch
   kSlotFrame1,
   kSlotFrame2,
-  kSlotBatchTables,  // batch offset tables
-  kSlotBatchWork,    // batch work lists / counters
+  kSlotBatchTables,  // batch offset tables (written by upload_table only)
+  kSlotBatchWork,    // batch counters, histograms, edges (kernel-written)
+  kSlotVolTables,    // batch volume tables (written by upload_table only)
+  kSlotTopoTables,   // batch topo tables (written by upload_table only)
   kSlotFix,          // auto-mode check: counter + list of points to recompute
   kSlotCluster0,     // cluster step: staged features / histograms
   kSlotCluster1,     // cluster step: edges, offsets, counts
@@ -121,6 +131,17 @@ int ctx_activate(cpet_ctx* ctx);
 // Copies host->device (or passes a device pointer through when dev != 0).
 int stage_in(cpet_ctx* ctx, int slot, const void* src, size_t bytes, bool dev,
              const void** out);
+// Host table -> device copy on the context's stream, skipped when the same
+// bytes were last uploaded to the same address under this key.  Overlapping
+// entries of other keys, and every entry when a scratch buffer is
+// reallocated, are forgotten.
+enum TableKey : int {
+  kTabPackX = 0, kTabPackC,
+  kTabVolGeo, kTabVolAx, kTabVolOut, kTabVolBlk,
+  kTabTopoHd, kTabTopoSl, kTabTopoAs, kTabTopoBlk,
+  kTabCount
+};
+int upload_table(cpet_ctx* ctx, int key, void* dst, const void* src, size_t bytes);
 // Global default context for the reference-compatible entry points.
 int default_ctx(cpet_ctx** out);
 

@@ -865,14 +865,14 @@ int field_batch_launch(cpet_ctx* ctx, int F, const int32_t* geo3, const float* a
   const size_t o_out = (o_ax + ax_off.size() * 4 + 255) & ~(size_t)255;
   const size_t o_blk = (o_out + out_off.size() * 8 + 255) & ~(size_t)255;
   void* work = nullptr;
-  int rc = ctx_buf(ctx, kSlotBatchWork, o_blk + blk.size() * 4 + 256, &work);
+  int rc = ctx_buf(ctx, kSlotVolTables, o_blk + blk.size() * 4 + 256, &work);
   if (rc) return rc;
   char* w = (char*)work;
   hipStream_t st = ctx->stream;
-  CPET_HIP_TRY(hipMemcpyAsync(w + o_geo, geo.data(), geo.size() * 4, hipMemcpyHostToDevice, st));
-  CPET_HIP_TRY(hipMemcpyAsync(w + o_ax, ax_off.data(), ax_off.size() * 4, hipMemcpyHostToDevice, st));
-  CPET_HIP_TRY(hipMemcpyAsync(w + o_out, out_off.data(), out_off.size() * 8, hipMemcpyHostToDevice, st));
-  CPET_HIP_TRY(hipMemcpyAsync(w + o_blk, blk.data(), blk.size() * 4, hipMemcpyHostToDevice, st));
+  if ((rc = upload_table(ctx, kTabVolGeo, w + o_geo, geo.data(), geo.size() * 4))) return rc;
+  if ((rc = upload_table(ctx, kTabVolAx, w + o_ax, ax_off.data(), ax_off.size() * 4))) return rc;
+  if ((rc = upload_table(ctx, kTabVolOut, w + o_out, out_off.data(), out_off.size() * 8))) return rc;
+  if ((rc = upload_table(ctx, kTabVolBlk, w + o_blk, blk.data(), blk.size() * 4))) return rc;
   BatchFieldArgs a;
   a.blk = (const int32_t*)(w + o_blk);
   a.geo = (const int32_t*)(w + o_geo);

@@ -718,21 +718,27 @@ int topo_batch_run(cpet_ctx* ctx, int F, const int32_t* n_starts, const float* s
     hd[4 * f + 3] = dims[3 * f + 2];
   }
   const int nbins = hist ? nb_d * nb_c : 0;
+  // read-only tables (cached uploads) in their own buffer ...
   const size_t o_hd = 0;
   const size_t o_sl = o_hd + hd.size() * sizeof(double);
-  const size_t o_cnt = (o_sl + sl_off.size() * sizeof(int32_t) + 255) & ~(size_t)255;
-  const size_t o_as = (o_cnt + (F + 1) * sizeof(uint32_t) + 255) & ~(size_t)255;
+  const size_t o_as = (o_sl + sl_off.size() * sizeof(int32_t) + 255) & ~(size_t)255;
   const size_t o_blk = (o_as + assign.size() * sizeof(int32_t) + 255) & ~(size_t)255;
-  const size_t o_hist = (o_blk + blk.size() * sizeof(int32_t) + 255) & ~(size_t)255;
+  const size_t t_total = o_blk + blk.size() * sizeof(int32_t) + 256;
+  void* tabs = nullptr;
+  if ((rc = ctx_buf(ctx, kSlotTopoTables, t_total, &tabs))) return rc;
+  char* tw = (char*)tabs;
+  if ((rc = upload_table(ctx, kTabTopoHd, tw + o_hd, hd.data(), hd.size() * sizeof(double)))) return rc;
+  if ((rc = upload_table(ctx, kTabTopoSl, tw + o_sl, sl_off.data(), sl_off.size() * sizeof(int32_t)))) return rc;
+  if ((rc = upload_table(ctx, kTabTopoAs, tw + o_as, assign.data(), assign.size() * sizeof(int32_t)))) return rc;
+  if ((rc = upload_table(ctx, kTabTopoBlk, tw + o_blk, blk.data(), blk.size() * sizeof(int32_t)))) return rc;
+  // ... and the counters, histograms and edges the call writes
+  const size_t o_cnt = 0;
+  const size_t o_hist = ((F + 1) * sizeof(uint32_t) + 255) & ~(size_t)255;
   const size_t o_edges = (o_hist + (size_t)F * nbins * sizeof(uint32_t) + 255) & ~(size_t)255;
   const size_t total = o_edges + (hist ? (size_t)(nb_d + nb_c + 2) * sizeof(double) : 0) + 256;
   void* work = nullptr;
   if ((rc = ctx_buf(ctx, kSlotBatchWork, total, &work))) return rc;
   char* w = (char*)work;
-  CPET_HIP_TRY(hipMemcpyAsync(w + o_hd, hd.data(), hd.size() * sizeof(double), hipMemcpyHostToDevice, st));
-  CPET_HIP_TRY(hipMemcpyAsync(w + o_sl, sl_off.data(), sl_off.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
-  CPET_HIP_TRY(hipMemcpyAsync(w + o_as, assign.data(), assign.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
-  CPET_HIP_TRY(hipMemcpyAsync(w + o_blk, blk.data(), blk.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
   unsigned int* d_cnt = (unsigned int*)(w + o_cnt);
   hipLaunchKernelGGL(fill_u32_kernel, dim3((F + 1 + 255) / 256), dim3(256), 0, st, d_cnt,
                      (long long)(F + 1), 0u);
@@ -760,15 +766,15 @@ int topo_batch_run(cpet_ctx* ctx, int F, const int32_t* n_starts, const float* s
   BatchTopoArgs ba;
   ba.F = F;
   ba.chg_off = d_chg_off;
-  ba.sl_off = (const int32_t*)(w + o_sl);
+  ba.sl_off = (const int32_t*)(tw + o_sl);
   ba.starts = (const float*)dstarts;
   ba.n_iter = (const int32_t*)diter;
-  ba.hdims = (const double*)(w + o_hd);
+  ba.hdims = (const double*)(tw + o_hd);
   ba.fin = d_fin;
   ba.steps = d_steps;
   ba.exited = d_exited;
   ba.sl_counter = d_cnt;
-  ba.assign = (const int32_t*)(w + o_as);
+  ba.assign = (const int32_t*)(tw + o_as);
   ba.n_assign = (int)assign.size();
   ba.assign_counter = d_cnt + F;
   if ((rc = prof_mark(ctx, 0))) return rc;
@@ -799,8 +805,8 @@ int topo_batch_run(cpet_ctx* ctx, int F, const int32_t* n_starts, const float* s
   ea.lds_hist = (hist && nbins <= 8192) ? 1 : 0;
   const size_t lds = ea.lds_hist ? (size_t)nbins * sizeof(unsigned int) : 0;
   hipLaunchKernelGGL(topo_batch_epilogue_kernel, dim3((unsigned)(blk.size() / 2)), dim3(256), lds,
-                     st, ch, ea, (const int32_t*)(w + o_blk), d_chg_off,
-                     (const int32_t*)(w + o_sl), (const double*)(w + o_hd));
+                     st, ch, ea, (const int32_t*)(tw + o_blk), d_chg_off,
+                     (const int32_t*)(tw + o_sl), (const double*)(tw + o_hd));
   CPET_HIP_TRY(hipGetLastError());
   if ((rc = prof_mark(ctx, 2))) return rc;
   if (!dev) {

@@ -11,7 +11,9 @@ per frame), inputs resident in HBM, timed three ways:
             slot refill;
   per-frame one launch per frame with the tuned single-frame kernels;
   batched   all volume frames in one launch and all streamline frames in one
-            persistent launch, on two streams.
+            persistent launch, on two streams;
+  batched_graph  the same two batched steps captured into HIP graphs
+            (pycpet_amd.graphs) and replayed on the two streams.
 serve: Poisson arrivals (pycpet_amd.serve.poisson_requests) served with
 continuous batching vs one request at a time (stopped after --plain-max-s,
 unserved requests counted with their latency lower bound); p50/p99 latency
@@ -84,8 +86,22 @@ def run_batch(n_frames: int, topo_starts: int, seed: int = 3) -> dict:
     B.batch_topo([f for f in small if f["kind"] == "topo"], ctx=ctx2)
     torch.cuda.synchronize()
 
+    # the same batched step captured into two HIP graphs (one per stream)
+    from . import graphs as G
+
+    gv = G.CapturedStep(lambda c: B.batch_volume(vol, ctx=c))
+    gt = G.CapturedStep(lambda c: B.batch_topo(top, ctx=c))
+
+    def batched_graph():
+        with torch.cuda.stream(s1):
+            gv.replay()
+        with torch.cuda.stream(s2):
+            gt.replay()
+        torch.cuda.synchronize()
+
     timings = {}
-    for name, fn in (("batched", batched), ("per_frame", lambda: per_frame(False)),
+    for name, fn in (("batched", batched), ("batched_graph", batched_graph),
+                     ("per_frame", lambda: per_frame(False)),
                      ("plain", lambda: per_frame(True))):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -104,7 +120,12 @@ def run_batch(n_frames: int, topo_starts: int, seed: int = 3) -> dict:
     for k, v in timings.items():
         out[f"{k}_s"] = v
         out[f"{k}_frames_per_s"] = n_frames / v
+    # the graph replay recomputes the eager batched results exactly
+    same = all(torch.equal(a[0], b[0]) for a, b in zip(gv.outputs, results["vol"])) and \
+        all(torch.equal(a.feats, b.feats) for a, b in zip(gt.outputs, results["top"]))
+    out["graph_equals_eager"] = bool(same)
     out["batched_vs_plain"] = timings["plain"] / timings["batched"]
+    out["graph_vs_plain"] = timings["plain"] / timings["batched_graph"]
     out["batched_vs_per_frame"] = timings["per_frame"] / timings["batched"]
     out["batched_pair_evals_per_s"] = (vpairs + tpairs) / timings["batched"]
     return out
