@@ -4,8 +4,9 @@ Requests (volume / volume_ESP / topo frames) arrive over time.  The server
 keeps `n_lanes` HIP streams (each with its own libcpet_hip context).  Every
 time a lane is free it takes all admitted requests that fit the HBM budget
 (and `max_batch`), copies their inputs to the device, and launches one batch
-call per request kind (cpet_batch_volume / cpet_batch_topo) followed by
-asynchronous copies of the results into pinned host memory.  Requests that
+call per request kind (cpet_batch_volume / cpet_batch_topo), each followed
+by asynchronous copies of its results into pinned host memory and an event:
+a kind's requests complete when their own event fires.  Requests that
 arrive while other batches run are therefore admitted into the next free
 lane and their work overlaps on the device with the running batches.
 Latency is measured from a request's arrival to its results being in host
@@ -89,16 +90,11 @@ class _Lane:
         self.torch = torch
         self.stream = torch.cuda.Stream(device=device)  # default priority
         self.ctx = nat.Context(device)
-        self.reqs: List[Request] = []
-        self.event = None
-        self.outputs = []
+        self.groups = collections.deque()  # (event, [(request, pinned result)])
 
     @property
     def busy(self) -> bool:
-        return bool(self.reqs)
-
-    def done(self) -> bool:
-        return self.event is not None and self.event.query()
+        return bool(self.groups)
 
 
 class Server:
@@ -115,52 +111,55 @@ class Server:
 
     # -------------------------------------------------------------- batching
     def _launch(self, lane: _Lane, reqs: List[Request]) -> None:
+        """Queues one batch call per request kind on the lane's stream, each
+        followed by its results' copies into pinned memory and an event, so
+        a kind's requests complete as soon as their own work is done."""
         torch = self.torch
         T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(self.dev, non_blocking=True)
-        lane.reqs = reqs
-        lane.outputs = []
-        with torch.cuda.stream(lane.stream):
-            vol = [r for r in reqs if r.kind == "volume"]
-            esp = [r for r in reqs if r.kind == "volume_ESP"]
-            top = [r for r in reqs if r.kind == "topo"]
-            if vol:
-                outs = B.batch_volume([{"x": T(r.data["x"]), "q": T(r.data["q"]),
-                                        "axes": tuple(T(a) for a in r.data["axes"])} for r in vol],
-                                      want_E=True, ctx=lane.ctx)
-                for r, (E, _) in zip(vol, outs):
-                    h = torch.empty(E.shape, dtype=E.dtype, pin_memory=True)
-                    h.copy_(E, non_blocking=True)
-                    lane.outputs.append((r, h))
-            if esp:
-                outs = B.batch_volume([{"x": T(r.data["x"]), "q": T(r.data["q"]),
-                                        "axes": tuple(T(a) for a in r.data["axes"])} for r in esp],
-                                      want_E=False, want_phi=True, ctx=lane.ctx)
-                for r, (_, phi) in zip(esp, outs):
-                    h = torch.empty(phi.shape, dtype=phi.dtype, pin_memory=True)
-                    h.copy_(phi, non_blocking=True)
-                    lane.outputs.append((r, h))
-            if top:
-                outs = B.batch_topo([{"x": T(r.data["x"]), "q": T(r.data["q"]),
-                                      "starts": T(r.data["starts"]), "n_iter": T(r.data["n_iter"]),
-                                      "h": r.data["h"], "dims": r.data["dims"]} for r in top],
-                                    ctx=lane.ctx)
-                for r, res in zip(top, outs):
-                    h = torch.empty(res.feats.shape, dtype=res.feats.dtype, pin_memory=True)
-                    h.copy_(res.feats, non_blocking=True)
-                    lane.outputs.append((r, h))
-            lane.event = torch.cuda.Event()
-            lane.event.record(lane.stream)
 
-    def _finish(self, lane: _Lane, now: float, stats: ServeStats) -> None:
-        for r, h in lane.outputs:
-            r.result = h.numpy()
-            r.t_done = now
-            stats.latencies.append(r.t_done - r.arrival)
-            stats.completed += 1
-            self.inflight -= r.nbytes()
-        lane.reqs = []
-        lane.outputs = []
-        lane.event = None
+        def pinned(t):
+            h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+            h.copy_(t, non_blocking=True)
+            return h
+
+        def close_group(outputs):
+            ev = torch.cuda.Event()
+            ev.record(lane.stream)
+            lane.groups.append((ev, outputs))
+
+        with torch.cuda.stream(lane.stream):
+            for kind in ("volume", "volume_ESP", "topo"):
+                grp = [r for r in reqs if r.kind == kind]
+                if not grp:
+                    continue
+                if kind == "topo":
+                    outs = B.batch_topo([{"x": T(r.data["x"]), "q": T(r.data["q"]),
+                                          "starts": T(r.data["starts"]),
+                                          "n_iter": T(r.data["n_iter"]), "h": r.data["h"],
+                                          "dims": r.data["dims"]} for r in grp], ctx=lane.ctx)
+                    close_group([(r, pinned(res.feats)) for r, res in zip(grp, outs)])
+                else:
+                    outs = B.batch_volume([{"x": T(r.data["x"]), "q": T(r.data["q"]),
+                                            "axes": tuple(T(a) for a in r.data["axes"])}
+                                           for r in grp], want_E=kind == "volume",
+                                          want_phi=kind == "volume_ESP", ctx=lane.ctx)
+                    close_group([(r, pinned(E if kind == "volume" else phi))
+                                 for r, (E, phi) in zip(grp, outs)])
+
+    def _finish_ready(self, lane: _Lane, t0: float, stats: ServeStats) -> int:
+        """Completes the lane's groups whose events have fired, in order."""
+        n = 0
+        while lane.groups and lane.groups[0][0].query():
+            _, outputs = lane.groups.popleft()
+            now = time.perf_counter() - t0
+            for r, h in outputs:
+                r.result = h.numpy()
+                r.t_done = now
+                stats.latencies.append(r.t_done - r.arrival)
+                stats.completed += 1
+                self.inflight -= r.nbytes()
+                n += 1
+        return n
 
     def run(self, requests: List[Request], stats: Optional[ServeStats] = None,
             poll_s: float = 2e-4) -> ServeStats:
@@ -181,10 +180,7 @@ class Server:
                     continue
                 pending.append(r)
             for lane in self.lanes:
-                if lane.busy and lane.done():
-                    n = len(lane.reqs)
-                    self._finish(lane, time.perf_counter() - t0, stats)
-                    remaining -= n
+                remaining -= self._finish_ready(lane, t0, stats)
             for lane in self.lanes:
                 if lane.busy or not pending:
                     continue
