@@ -38,6 +38,22 @@ PEAK_FP32_UNPACKED_TFLOPS = 78.6  # without packed FP32 (disabled in this build)
 FLOP_PER_PAIR = 18
 
 
+TRAFFIC_FILE = ROOT / "profiles" / "pmc_traffic_topo5k.txt"
+
+
+def measured_traffic():
+    """HBM bytes per topo_propagate launch from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by the same
+    counter's reading for a plain copy of known size), or None."""
+    try:
+        for line in TRAFFIC_FILE.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(case, gpu_steps, seconds: float, cores: int):
     """Reference thread_operation on a seeded sample of the same streamlines."""
     from oracle import oracle as O
@@ -154,7 +170,8 @@ def main() -> int:
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
             "frac_of_unpacked_peak": achieved / PEAK_FP32_UNPACKED_TFLOPS,
-            "traffic": None,
+            "traffic": measured_traffic(),
+            "traffic_unit": "bytes per launch (rocprofv3 PMC, copy-calibrated)",
             "kernel_ms": prop,
             "epilogue_ms": float(np.mean(epi_ms)),
             "flop_per_pair": FLOP_PER_PAIR,

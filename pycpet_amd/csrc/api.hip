@@ -128,6 +128,15 @@ int upload_table(cpet_ctx* ctx, int key, void* dst, const void* src, size_t byte
     auto& o = ctx->tables[k];
     if (k != key && o.dst && o.dst < d + bytes && d < o.dst + o.data.size()) o.dst = nullptr;
   }
+  // a copy from host memory must not be recorded into a graph: the graph
+  // would read this host buffer again at every replay
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  CPET_HIP_TRY(hipStreamIsCapturing(ctx->stream, &cs));
+  if (cs != hipStreamCaptureStatusNone) {
+    set_error("batch tables changed while the stream is being captured: run the step once "
+              "eagerly on this context, with the same shapes, before capturing it");
+    return CPET_ERR_ARG;
+  }
   t.dst = nullptr;
   t.data.assign((const char*)src, (const char*)src + bytes);
   if (bytes)
@@ -179,7 +188,7 @@ int pack_charges(cpet_ctx* ctx, int N, const float* x, const float* q,
   return CPET_OK;
 }
 
-int pack_charges_batch(cpet_ctx* ctx, int F, const int32_t* nch, const float* x,
+int pack_charges_batch(cpet_ctx* ctx, int which, int F, const int32_t* nch, const float* x,
                        const float* q, bool dev, const float4** out,
                        const double4** out64, std::vector<int32_t>& chg_off,
                        const int32_t** d_chg_off) {
@@ -208,12 +217,15 @@ int pack_charges_batch(cpet_ctx* ctx, int F, const int32_t* nch, const float* x,
   if (rc) return rc;
   void* tab = nullptr;
   const size_t tab_bytes = (size_t)(F + 1) * (sizeof(int32_t) + sizeof(int64_t)) + 256;
-  if ((rc = ctx_buf(ctx, kSlotBatchTables, tab_bytes, &tab))) return rc;
+  if ((rc = ctx_buf(ctx, which ? kSlotBatchTables2 : kSlotBatchTables, tab_bytes, &tab)))
+    return rc;
   int64_t* d_xoff = (int64_t*)tab;
   int32_t* d_coff = (int32_t*)(d_xoff + F + 1);
-  if ((rc = upload_table(ctx, kTabPackX, d_xoff, x_off.data(), (F + 1) * sizeof(int64_t))))
+  if ((rc = upload_table(ctx, which ? kTabPackX2 : kTabPackX, d_xoff, x_off.data(),
+                         (F + 1) * sizeof(int64_t))))
     return rc;
-  if ((rc = upload_table(ctx, kTabPackC, d_coff, chg_off.data(), (F + 1) * sizeof(int32_t))))
+  if ((rc = upload_table(ctx, which ? kTabPackC2 : kTabPackC, d_coff, chg_off.data(),
+                         (F + 1) * sizeof(int32_t))))
     return rc;
   void* d = nullptr;
   if ((rc = ctx_buf(ctx, kSlotCharges, (size_t)total * sizeof(float4), &d))) return rc;

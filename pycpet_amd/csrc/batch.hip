@@ -48,7 +48,7 @@ int cpet_batch_volume(cpet_ctx* ctx, int F, const int32_t* n_charges, const floa
   const double4* chd = nullptr;
   std::vector<int32_t> chg_off;
   const int32_t* d_chg_off = nullptr;
-  if ((rc = pack_charges_batch(ctx, F, n_charges, x, q, dev, &ch, &chd, chg_off, &d_chg_off)))
+  if ((rc = pack_charges_batch(ctx, 0, F, n_charges, x, q, dev, &ch, &chd, chg_off, &d_chg_off)))
     return rc;
   const void* daxes = nullptr;
   if ((rc = stage_in(ctx, kSlotAxes, axes, (size_t)naxes * sizeof(float), dev, &daxes))) return rc;
@@ -95,7 +95,7 @@ int cpet_batch_topo(cpet_ctx* ctx, int F, const int32_t* n_charges, const float*
   const float4* ch = nullptr;
   std::vector<int32_t> chg_off;
   const int32_t* d_chg_off = nullptr;
-  if ((rc = pack_charges_batch(ctx, F, n_charges, x, q, dev, &ch, nullptr, chg_off, &d_chg_off)))
+  if ((rc = pack_charges_batch(ctx, 1, F, n_charges, x, q, dev, &ch, nullptr, chg_off, &d_chg_off)))
     return rc;
   return topo_batch_run(ctx, F, n_starts, starts, n_iter, h, dims, ch, d_chg_off, chg_off,
                         feats, steps, exited, nb_d, edges_d, nb_c, edges_c, hist, flags);

@@ -113,7 +113,8 @@ enum Slot : int {
   kSlotFrame0,       // frame-transform scratch
   kSlotFrame1,
   kSlotFrame2,
-  kSlotBatchTables,  // batch offset tables (written by upload_table only)
+  kSlotBatchTables,  // volume batch charge offsets (written by upload_table only)
+  kSlotBatchTables2, // topo batch charge offsets (written by upload_table only)
   kSlotBatchWork,    // batch counters, histograms, edges (kernel-written)
   kSlotVolTables,    // batch volume tables (written by upload_table only)
   kSlotTopoTables,   // batch topo tables (written by upload_table only)
@@ -136,7 +137,7 @@ int stage_in(cpet_ctx* ctx, int slot, const void* src, size_t bytes, bool dev,
 // entries of other keys, and every entry when a scratch buffer is
 // reallocated, are forgotten.
 enum TableKey : int {
-  kTabPackX = 0, kTabPackC,
+  kTabPackX = 0, kTabPackC, kTabPackX2, kTabPackC2,
   kTabVolGeo, kTabVolAx, kTabVolOut, kTabVolBlk,
   kTabTopoHd, kTabTopoSl, kTabTopoAs, kTabTopoBlk,
   kTabCount
@@ -157,7 +158,9 @@ inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
 // padded float4 buffer (and a double4 copy when out64 is non-null); each
 // frame starts at chg_off[f] (a multiple of kTile).  chg_off (F+1) is filled
 // on the host; *d_chg_off points at its device copy.
-int pack_charges_batch(cpet_ctx* ctx, int F, const int32_t* nch, const float* x,
+// `which` (0 volume, 1 topo) keeps the two batch kinds' offset tables apart,
+// so one step may run both and still re-upload nothing.
+int pack_charges_batch(cpet_ctx* ctx, int which, int F, const int32_t* nch, const float* x,
                        const float* q, bool dev, const float4** out,
                        const double4** out64, std::vector<int32_t>& chg_off,
                        const int32_t** d_chg_off);
