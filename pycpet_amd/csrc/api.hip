@@ -96,7 +96,10 @@ int ctx_buf(cpet_ctx* ctx, int slot, size_t bytes, void** out) {
   DevBuf& b = ctx->bufs[slot];
   if (bytes == 0) bytes = 256;
   if (b.bytes < bytes) {
-    for (auto& t : ctx->tables) t.dst = nullptr;  // cached tables may move
+    // tables cached in this buffer are lost with it
+    for (auto& t : ctx->tables)
+      if (t.dst && b.ptr && t.dst >= (const char*)b.ptr && t.dst < (const char*)b.ptr + b.bytes)
+        t.dst = nullptr;
     if (b.ptr) {
       // the old buffer may still be read by queued work
       CPET_HIP_TRY(hipStreamSynchronize(ctx->stream));

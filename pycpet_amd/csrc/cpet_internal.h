@@ -134,8 +134,8 @@ int stage_in(cpet_ctx* ctx, int slot, const void* src, size_t bytes, bool dev,
              const void** out);
 // Host table -> device copy on the context's stream, skipped when the same
 // bytes were last uploaded to the same address under this key.  Overlapping
-// entries of other keys, and every entry when a scratch buffer is
-// reallocated, are forgotten.
+// entries of other keys, and the entries inside a scratch buffer that is
+// reallocated, are forgotten.  Fails while the stream is being captured.
 enum TableKey : int {
   kTabPackX = 0, kTabPackC, kTabPackX2, kTabPackC2,
   kTabVolGeo, kTabVolAx, kTabVolOut, kTabVolBlk,
