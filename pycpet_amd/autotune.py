@@ -114,3 +114,44 @@ def tune_grid(ctx: nat.Context, axes, x, q, *, want_phi=False, cache: Optional[d
         cache[k] = list(best)
     ctx.set_tuning(0, *best)
     return best
+
+
+def tune_batch_topo(ctx: nat.Context, frames, *, cache: Optional[dict] = None,
+                    waves=(4, 8, 16), groups=(0, 512, 1024, 2048)) -> Tuple[int, int]:
+    """(waves per workgroup, persistent workgroups) of the batched
+    streamline launch, timed on `frames` (a representative subset)."""
+    from . import batch as B
+
+    n = int(np.mean([int(np.prod(f["q"].shape)) for f in frames]))
+    L = int(sum(int(f["starts"].shape[0]) for f in frames))
+    k = key(3, n, L)
+    if cache is not None and k in cache:
+        a, b = cache[k]
+        ctx.set_tuning(3, a, b)
+        return a, b
+    cands = [(w, g) for w in waves for g in groups]
+    best, _ = sweep(ctx, 3, cands, lambda: B.batch_topo(frames, ctx=ctx), phase=0)
+    if cache is not None:
+        cache[k] = list(best)
+    ctx.set_tuning(3, *best)
+    return best
+
+
+def tune_batch_volume(ctx: nat.Context, frames, *, cache: Optional[dict] = None,
+                      ks=(1, 2, 4)) -> Tuple[int, int]:
+    """Points per lane of the batched grid launch, timed on `frames`."""
+    from . import batch as B
+
+    n = int(np.mean([int(np.prod(f["q"].shape)) for f in frames]))
+    m = int(sum(int(np.prod([len(a) for a in f["axes"]])) for f in frames))
+    k = key(4, n, m)
+    if cache is not None and k in cache:
+        a, b = cache[k]
+        ctx.set_tuning(4, a, b)
+        return a, b
+    best, _ = sweep(ctx, 4, [(kk, 0) for kk in ks], lambda: B.batch_volume(frames, ctx=ctx),
+                    phase=0)
+    if cache is not None:
+        cache[k] = list(best)
+    ctx.set_tuning(4, *best)
+    return best

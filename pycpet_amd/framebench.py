@@ -86,11 +86,20 @@ def run_batch(n_frames: int, topo_starts: int, seed: int = 3) -> dict:
     B.batch_topo([f for f in small if f["kind"] == "topo"], ctx=ctx2)
     torch.cuda.synchronize()
 
+    # launch shapes of the batched calls from the autotuner, timed on a subset
+    from . import autotune as AT
+
+    kv = AT.tune_batch_volume(ctx, vol[:8])
+    kt = AT.tune_batch_topo(ctx2, top[:4])
+    ctx.set_profiling(False)
+    ctx2.set_profiling(False)
+    print(f"autotuned: batch volume K={kv[0]}, batch topo S,G={kt}", file=sys.stderr, flush=True)
+
     # the same batched step captured into two HIP graphs (one per stream)
     from . import graphs as G
 
-    gv = G.CapturedStep(lambda c: B.batch_volume(vol, ctx=c))
-    gt = G.CapturedStep(lambda c: B.batch_topo(top, ctx=c))
+    gv = G.CapturedStep(lambda c: B.batch_volume(vol, ctx=c), tuning={4: kv})
+    gt = G.CapturedStep(lambda c: B.batch_topo(top, ctx=c), tuning={3: kt})
 
     def batched_graph():
         with torch.cuda.stream(s1):
@@ -115,6 +124,7 @@ def run_batch(n_frames: int, topo_starts: int, seed: int = 3) -> dict:
     tpairs = sum((int(r.steps.sum().item()) + 4 * r.steps.numel()) * f["q"].numel()
                  for r, f in zip(results["top"], top))
     out = {"workload": "md_frames", "frames": n_frames, "volume_frames": len(vol),
+           "autotuned": {"batch_volume_K": kv[0], "batch_topo_S_G": list(kt)},
            "topo_frames": len(top), "topo_starts_per_frame": topo_starts,
            "pair_evals": vpairs + tpairs}
     for k, v in timings.items():

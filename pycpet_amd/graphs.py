@@ -21,18 +21,21 @@ counts and grid / streamline counts are fixed by the capture.
 """
 from __future__ import annotations
 
-from typing import Any, Callable
+from typing import Any, Callable, Dict, Optional, Tuple
 
 from . import _native as nat
 
 
 class CapturedStep:
-    def __init__(self, fn: Callable[[nat.Context], Any], device: int = 0, warmup: int = 1):
+    def __init__(self, fn: Callable[[nat.Context], Any], device: int = 0, warmup: int = 1,
+                 tuning: Optional[Dict[int, Tuple[int, int]]] = None):
         import torch
 
         self.torch = torch
         self.ctx = nat.Context(device)
         self.ctx.set_profiling(False)
+        for kind, (a, b) in (tuning or {}).items():  # launch shapes are fixed at capture
+            self.ctx.set_tuning(kind, a, b)
         side = torch.cuda.Stream(device=device)
         side.wait_stream(torch.cuda.current_stream(device))
         with torch.cuda.stream(side):
