@@ -41,6 +41,7 @@ extern "C" {
 #define CPET_F_DEVICE 0x1u   /* array arguments are device pointers */
 #define CPET_F_OUT_F64 0x2u  /* field/phi outputs are double, not float */
 #define CPET_F_NO_SYNC 0x4u  /* device-pointer calls: do not wait for the stream */
+#define CPET_F_DIPOLE_PHYSICAL 0x10u /* cpet_topo_dipole: physical 3(mu.R)R - mu r^2 term */
 /* precision of the field kernels, bits 8..10 */
 #define CPET_F_PREC_AUTO (0u << 8)  /* E: FP32 tiled, points with strong
                                        cancellation recomputed as MIXED;
@@ -89,6 +90,16 @@ int calc_field_base(float *E, const float *x_init, int n_charges,
 int thread_operation(int n_charges, int n_iter, float step_size,
                      const float *x_0, const float *dimensions,
                      const float *x, const float *Q, float *ret);
+
+/* Dipole streamline from x_0 (the dipole variant of thread_operation,
+ * marked "in development" in the reference): the field of n_dipoles point
+ * dipoles x (n,3) with moments mu (n,3) as calc_field_base_dipole computes
+ * it, E = k sum r^-5 ((3 mu_x R_x + mu_y R_y + mu_z R_z) R - mu r^2).
+ * Replaces thread_operation_dipole, CPET/utils/math_module.c:593-659
+ * (binding CPET/utils/c_ops.py:97-110, wrapper :304-325). */
+int thread_operation_dipole(int n_dipoles, int n_iter, float step_size, const float *x_0,
+                            const float *dimensions, const float *x, const float *mu,
+                            float *ret);
 
 /* ============ Part 2: batched / device-resident API ====================== */
 
@@ -154,6 +165,15 @@ int cpet_topo(cpet_ctx *ctx, int N, int L, const float *starts,
               const float *x, const float *q, double *feats, int32_t *steps,
               int32_t *exited, int nb_d, const double *edges_d, int nb_c,
               const double *edges_c, uint32_t *hist, uint32_t flags);
+
+/* L dipole streamlines: cpet_topo with point dipoles x (N,3), mu (N,3) as
+ * the sources (calc_field_base_dipole, math_module.c:335-371; walk and
+ * features of thread_operation_dipole, :593-659).  The pair term is the
+ * reference's unless flags has CPET_F_DIPOLE_PHYSICAL (3 (mu.R) R - mu r^2).
+ * dims is a host array; the other arrays follow CPET_F_DEVICE. */
+int cpet_topo_dipole(cpet_ctx *ctx, int N, int L, const float *starts, const int32_t *n_iter,
+                     float h, const float *dims, const float *x, const float *mu,
+                     double *feats, int32_t *steps, int32_t *exited, uint32_t flags);
 
 /* ---- many frames per launch (batching) ---------------------------------
  * Per-frame sizes and scalars (n_charges, n_starts, grid_dims, h, dims)

@@ -96,12 +96,49 @@ static void field_at_f64(const double p[3], int N, const float *x,
     E[2] = ORACLE_K * ez;
 }
 
-/* One Euler step along the unit field direction (math_module.c:489-503). */
-static void step_f64(const double p[3], double out[3], int N, const float *x,
-                     const float *q, double h)
+/* Field of the N sources at p: charges (src = q) or dipoles (src = mu). */
+typedef void (*field_fn)(const double p[3], int N, const float *x, const float *src, int opt,
+                         double E[3]);
+
+static void charge_field(const double p[3], int N, const float *x, const float *q, int opt,
+                         double E[3])
+{
+    (void)opt;
+    field_at_f64(p, N, x, q, E);
+}
+
+/* Field of N point dipoles (calc_field_base_dipole, math_module.c:335-371):
+ *   E = k sum_j r^-5 (c R - mu_j r^2),  R = p - x_j,
+ * with c = 3 mu_x R_x + mu_y R_y + mu_z R_z as the reference writes it
+ * (opt = 0), or the physical c = 3 (mu . R) (opt = 1). */
+static void dipole_field(const double p[3], int N, const float *x, const float *mu, int opt,
+                         double E[3])
+{
+    double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+    for (int j = 0; j < N; ++j) {
+        const double R0 = p[0] - x[3 * j], R1 = p[1] - x[3 * j + 1], R2 = p[2] - x[3 * j + 2];
+        const double m0 = mu[3 * j], m1 = mu[3 * j + 1], m2 = mu[3 * j + 2];
+        const double r2 = R0 * R0 + R1 * R1 + R2 * R2;
+        const double r = sqrt(r2);
+        const double r5 = 1.0 / (r2 * r2 * r);
+        const double c = opt ? 3.0 * (m0 * R0 + m1 * R1 + m2 * R2)
+                             : 3.0 * m0 * R0 + m1 * R1 + m2 * R2;
+        e0 += r5 * (c * R0 - m0 * r2);
+        e1 += r5 * (c * R1 - m1 * r2);
+        e2 += r5 * (c * R2 - m2 * r2);
+    }
+    E[0] = ORACLE_K * e0;
+    E[1] = ORACLE_K * e1;
+    E[2] = ORACLE_K * e2;
+}
+
+/* One Euler step along the unit field direction (math_module.c:489-503,
+ * dipoles :506-520). */
+static void step_f64(field_fn fld, int opt, const double p[3], double out[3], int N,
+                     const float *x, const float *src, double h)
 {
     double E[3];
-    field_at_f64(p, N, x, q, E);
+    fld(p, N, x, src, opt, E);
     const double n = sqrt(E[0] * E[0] + E[1] * E[1] + E[2] * E[2]);
     out[0] = p[0] + h * E[0] / n;
     out[1] = p[1] + h * E[1] / n;
@@ -124,10 +161,10 @@ static double curve_f64(const double a[3], const double b[3])
  *   steps[l]    = number of Euler steps taken (<= n_iter[l])
  *   exited[l]   = 1 if the walk ended on a box exit, 0 on the step limit
  *   pts6 (opt.) = the six points x0, x0+, x0++, xf, xf+, xf++ (L x 6 x 3)  */
-void oracle_topo_f64(int N, int L, const float *starts, const int32_t *n_iter,
-                     float h32, const float *dims, const float *x,
-                     const float *q, double *feats, int32_t *steps,
-                     int32_t *exited, double *pts6)
+static void walk_f64(field_fn fld, int opt, int N, int L, const float *starts,
+                     const int32_t *n_iter, float h32, const float *dims, const float *x,
+                     const float *q, double *feats, int32_t *steps, int32_t *exited,
+                     double *pts6)
 {
     const double h = (double)h32;
     const double d0 = dims[0], d1 = dims[1], d2 = dims[2];
@@ -138,7 +175,7 @@ void oracle_topo_f64(int N, int L, const float *starts, const int32_t *n_iter,
         int s = 0, out = 0;
         for (int i = 0; i < n_iter[l]; ++i) {
             double nx[3];
-            step_f64(p, nx, N, x, q, h);
+            step_f64(fld, opt, p, nx, N, x, q, h);
             p[0] = nx[0];
             p[1] = nx[1];
             p[2] = nx[2];
@@ -150,10 +187,10 @@ void oracle_topo_f64(int N, int L, const float *starts, const int32_t *n_iter,
             }
         }
         double a1[3], a2[3], b1[3], b2[3];
-        step_f64(x0, a1, N, x, q, h);
-        step_f64(a1, a2, N, x, q, h);
-        step_f64(p, b1, N, x, q, h);
-        step_f64(b1, b2, N, x, q, h);
+        step_f64(fld, opt, x0, a1, N, x, q, h);
+        step_f64(fld, opt, a1, a2, N, x, q, h);
+        step_f64(fld, opt, p, b1, N, x, q, h);
+        step_f64(fld, opt, b1, b2, N, x, q, h);
         double v1[3], v2[3], w1[3], w2[3];
         for (int i = 0; i < 3; ++i) {
             v1[i] = a1[i] - x0[i];
@@ -177,6 +214,25 @@ void oracle_topo_f64(int N, int L, const float *starts, const int32_t *n_iter,
                     pts6[(size_t)l * 18 + 3 * k + i] = src[k][i];
         }
     }
+}
+
+void oracle_topo_f64(int N, int L, const float *starts, const int32_t *n_iter,
+                     float h32, const float *dims, const float *x,
+                     const float *q, double *feats, int32_t *steps,
+                     int32_t *exited, double *pts6)
+{
+    walk_f64(charge_field, 0, N, L, starts, n_iter, h32, dims, x, q, feats, steps, exited, pts6);
+}
+
+/* Dipole streamlines (thread_operation_dipole, math_module.c:593-659): the
+ * same walk and features with the dipole field; physical selects the field
+ * form (see dipole_field). */
+void oracle_topo_dipole_f64(int N, int L, const float *starts, const int32_t *n_iter,
+                            float h32, const float *dims, const float *x, const float *mu,
+                            int physical, double *feats, int32_t *steps, int32_t *exited)
+{
+    walk_f64(dipole_field, physical, N, L, starts, n_iter, h32, dims, x, mu, feats, steps,
+             exited, NULL);
 }
 
 /* ------------------------------------------------------------------ FP32 */

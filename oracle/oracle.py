@@ -51,7 +51,9 @@ def lib():
         L.oracle_field_f32.argtypes = [_i, _i, _P, _P, _P, _i, _P, _P]
         L.oracle_topo_f64.argtypes = [_i, _i, _P, _P, _f, _P, _P, _P, _P, _P, _P, _P]
         L.oracle_topo_f32.argtypes = [_i, _i, _P, _P, _f, _P, _P, _P, _P, _P]
-        for fn in (L.oracle_field_f64, L.oracle_field_f32, L.oracle_topo_f64, L.oracle_topo_f32):
+        L.oracle_topo_dipole_f64.argtypes = [_i, _i, _P, _P, _f, _P, _P, _P, _i, _P, _P, _P]
+        for fn in (L.oracle_field_f64, L.oracle_field_f32, L.oracle_topo_f64, L.oracle_topo_f32,
+                   L.oracle_topo_dipole_f64):
             fn.restype = None
         _lib = L
     return _lib
@@ -110,6 +112,27 @@ def topo_f64(starts, n_iter, h, dims, x, q, *, want_points=False):
     return feats, steps, exited
 
 
+def topo_dipole_f64(starts, n_iter, h, dims, x, mu, *, physical=False):
+    """FP64 dipole streamlines (thread_operation_dipole, math_module.c:
+    593-659): the walk of topo_f64 with the dipole field of
+    calc_field_base_dipole (math_module.c:335-371), in the reference's form
+    c = 3 mu_x R_x + mu_y R_y + mu_z R_z, or the physical c = 3 mu.R.
+    Returns feats (L,2), steps (L,), exited (L,).  Pinned by
+    tests/test_oracle_topo.py (reference-C thread_operation_dipole vectors,
+    the on-axis closed form E = 2 k mu / r^3 with exact exit steps, and the
+    two forms agreeing for dipoles along x)."""
+    starts, x, mu, dims = _f32(starts, 3), _f32(x, 3), _f32(mu, 3), _f32(dims)
+    n_iter = np.ascontiguousarray(n_iter, dtype=np.int32).ravel()
+    L, N = starts.shape[0], x.shape[0]
+    feats = np.zeros((L, 2))
+    steps = np.zeros(L, np.int32)
+    exited = np.zeros(L, np.int32)
+    lib().oracle_topo_dipole_f64(N, L, _p(starts), _p(n_iter), float(np.float32(h)), _p(dims),
+                                 _p(x), _p(mu), 1 if physical else 0, _p(feats), _p(steps),
+                                 _p(exited))
+    return feats, steps, exited
+
+
 def topo_f32(starts, n_iter, h, dims, x, q):
     """FP32 port of thread_operation (restated CPU baseline)."""
     starts, x, q, dims = _f32(starts, 3), _f32(x, 3), _f32(q), _f32(dims)
@@ -137,8 +160,9 @@ class Ref:
         L.calc_field.argtypes = [_P, _P, _i, _P, _P]
         L.calc_field_base.argtypes = [_P, _P, _i, _P, _P]
         L.thread_operation.argtypes = [_i, _i, _f, _P, _P, _P, _P, _P]
+        L.thread_operation_dipole.argtypes = [_i, _i, _f, _P, _P, _P, _P, _P]
         for fn in (L.compute_looped_field, L.calc_esp_base, L.calc_field,
-                   L.calc_field_base, L.thread_operation):
+                   L.calc_field_base, L.thread_operation, L.thread_operation_dipole):
             fn.restype = None
         self.L = L
 
@@ -176,6 +200,15 @@ class Ref:
         ret = np.zeros(2, np.float32)
         self.L.thread_operation(q.size, int(n_iter), float(np.float32(h)), _p(start), _p(dims),
                                 _p(x), _p(q), _p(ret))
+        return ret
+
+    def thread_operation_dipole(self, start, n_iter, h, dims, x, mu):
+        """Argument order of the C function (math_module.c:593): n_dipoles,
+        n_iter, step_size, x_0, dimensions, x, mu, ret."""
+        start, x, mu, dims = _f32(start), _f32(x, 3), _f32(mu, 3), _f32(dims)
+        ret = np.zeros(2, np.float32)
+        self.L.thread_operation_dipole(x.shape[0], int(n_iter), float(np.float32(h)), _p(start),
+                                       _p(dims), _p(x), _p(mu), _p(ret))
         return ret
 
     def topo(self, starts, n_iter, h, dims, x, q):

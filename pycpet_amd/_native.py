@@ -28,6 +28,7 @@ CPET_ERR_BUDGET = -5
 F_DEVICE = 0x1
 F_OUT_F64 = 0x2
 F_NO_SYNC = 0x4
+F_DIPOLE_PHYSICAL = 0x10
 PREC = {"auto": 0 << 8, "naive": 1 << 8, "kahan": 2 << 8, "f64": 3 << 8,
         "tiled": 4 << 8, "mixed": 5 << 8}
 
@@ -61,6 +62,7 @@ SIGNATURES = {
     "calc_field": [P, P, c_int, P, P],
     "calc_field_base": [P, P, c_int, P, P],
     "thread_operation": [c_int, c_int, c_float, P, P, P, P, P],
+    "thread_operation_dipole": [c_int, c_int, c_float, P, P, P, P, P],
     # batched / device-resident
     "cpet_device_count": [P],
     "cpet_ctx_create": [c_int, P],
@@ -73,6 +75,7 @@ SIGNATURES = {
     "cpet_topo": [c_void_p, c_int, c_int, P, P, c_float, P, P, P, P, P, P,
                   c_int, P, c_int, P, P, c_uint32],
     "cpet_set_tuning": [c_void_p, c_int, c_int, c_int],
+    "cpet_topo_dipole": [c_void_p, c_int, c_int, P, P, c_float, P, P, P, P, P, P, c_uint32],
     "cpet_ctx_set_profiling": [c_void_p, c_int],
     "cpet_ctx_phase_times": [c_void_p, P, c_int],
     "cpet_ctx_refined_points": [c_void_p, P],

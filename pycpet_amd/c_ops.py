@@ -83,6 +83,20 @@ class Math_ops:
             dims.ctypes.data, x.ctypes.data, Q.ctypes.data, res.ctypes.data))
         return res
 
+    def thread_operation_dipole(self, x_0, n_iter, x, mu, step_size, dimensions):
+        """(2,) float32 [distance, mean curvature] of one streamline in the
+        field of point dipoles (CPET/utils/c_ops.py:304-325, argument order
+        kept)."""
+        x_0 = _f32(x_0)
+        x = _f32(x, 3)
+        mu = _f32(mu, 3)
+        dims = _f32(dimensions)
+        res = np.zeros(2, dtype="float32")
+        self._check(self.math.thread_operation_dipole(
+            len(mu), int(n_iter), ctypes.c_float(float(step_size)), x_0.ctypes.data,
+            dims.ctypes.data, x.ctypes.data, mu.ctypes.data, res.ctypes.data))
+        return res
+
     # Batched forms (no per-point / per-streamline FFI loop).
     def esp_points(self, x_0, x, Q):
         """phi at every row of x_0 in one launch (float32)."""

@@ -22,7 +22,7 @@ int topo_run(cpet_ctx* ctx, int N, int L, const float* starts,
              const int32_t* n_iter, float h, const float* dims, const float4* ch,
              int npad, double* feats, int32_t* steps, int32_t* exited, int nb_d,
              const double* edges_d, int nb_c, const double* edges_c,
-             uint32_t* hist, uint32_t flags);
+             uint32_t* hist, uint32_t flags, int src);
 
 namespace {
 thread_local char g_err[512] = "";
@@ -41,6 +41,22 @@ __global__ __launch_bounds__(256) void pack_charges_kernel(
     c = make_float4(kPadCoord, kPadCoord, kPadCoord, 0.f);
   out[j] = c;
   if (out64) out64[j] = make_double4(c.x, c.y, c.z, c.w);
+}
+
+// Dipoles: out[2j] = (x, y, z, 0), out[2j+1] = (mu, 0); padding dipoles have
+// mu = 0 at 1e18 (their terms are exactly 0).
+__global__ __launch_bounds__(256) void pack_dipoles_kernel(
+    int N, int npad, const float* __restrict__ x, const float* __restrict__ mu,
+    float4* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= npad) return;
+  if (j < N) {
+    out[2 * j] = make_float4(x[3 * j], x[3 * j + 1], x[3 * j + 2], 0.f);
+    out[2 * j + 1] = make_float4(mu[3 * j], mu[3 * j + 1], mu[3 * j + 2], 0.f);
+  } else {
+    out[2 * j] = make_float4(kPadCoord, kPadCoord, kPadCoord, 0.f);
+    out[2 * j + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
 }
 
 // Batched pack: thread j -> frame f (binary search on chg_off), row j - chg_off[f].
@@ -586,7 +602,42 @@ int cpet_topo(cpet_ctx* ctx, int N, int L, const float* starts,
   rc = pack_charges(ctx, N, x, q, dev, &ch, nullptr, &npad);
   if (rc) return rc;
   return topo_run(ctx, N, L, starts, n_iter, h, hdims, ch, npad, feats, steps,
-                  exited, nb_d, edges_d, nb_c, edges_c, hist, flags);
+                  exited, nb_d, edges_d, nb_c, edges_c, hist, flags, kSrcCharge);
+}
+
+int cpet_topo_dipole(cpet_ctx* ctx, int N, int L, const float* starts, const int32_t* n_iter,
+                     float h, const float* dims, const float* x, const float* mu,
+                     double* feats, int32_t* steps, int32_t* exited, uint32_t flags) {
+  if (!ctx || N <= 0 || L <= 0 || !starts || !n_iter || !dims || !x || !mu || !feats) {
+    set_error("cpet_topo_dipole: bad arguments (N=%d L=%d)", N, L);
+    return CPET_ERR_ARG;
+  }
+  if (!(std::isfinite(h) && h > 0.f)) {
+    set_error("step size must be finite and > 0 (got %g)", (double)h);
+    return CPET_ERR_ARG;
+  }
+  float hdims[3];
+  std::memcpy(hdims, dims, sizeof(hdims));
+  if (!finite3(hdims) || !(hdims[0] > 0.f && hdims[1] > 0.f && hdims[2] > 0.f)) {
+    set_error("box half-widths must be finite and > 0");
+    return CPET_ERR_ARG;
+  }
+  const bool dev = (flags & CPET_F_DEVICE) != 0;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  int rc = ctx_activate(ctx);
+  if (rc) return rc;
+  const int npad = round_up(N, kTile);
+  const void *dx = nullptr, *dmu = nullptr;
+  if ((rc = stage_in(ctx, kSlotX, x, (size_t)N * 3 * sizeof(float), dev, &dx))) return rc;
+  if ((rc = stage_in(ctx, kSlotQ, mu, (size_t)N * 3 * sizeof(float), dev, &dmu))) return rc;
+  void* d = nullptr;
+  if ((rc = ctx_buf(ctx, kSlotCharges, (size_t)npad * 2 * sizeof(float4), &d))) return rc;
+  hipLaunchKernelGGL(pack_dipoles_kernel, dim3((npad + 255) / 256), dim3(256), 0, ctx->stream, N,
+                     npad, (const float*)dx, (const float*)dmu, (float4*)d);
+  CPET_HIP_TRY(hipGetLastError());
+  const int src = (flags & CPET_F_DIPOLE_PHYSICAL) ? kSrcDipolePhys : kSrcDipoleRef;
+  return topo_run(ctx, N, L, starts, n_iter, h, hdims, (const float4*)d, npad, feats, steps,
+                  exited, 0, nullptr, 0, nullptr, nullptr, flags, src);
 }
 
 // ---------------------------------------------------------------------------
@@ -658,6 +709,26 @@ int thread_operation(int n_charges, int n_iter, float step_size,
   const int32_t it = n_iter;
   rc = cpet_topo(c, n_charges, 1, x_0, &it, step_size, dimensions, x, Q, f,
                  nullptr, nullptr, 0, nullptr, 0, nullptr, nullptr, 0u);
+  if (rc) return rc;
+  ret[0] = (float)f[0];
+  ret[1] = (float)f[1];
+  return CPET_OK;
+}
+
+int thread_operation_dipole(int n_dipoles, int n_iter, float step_size, const float* x_0,
+                            const float* dimensions, const float* x, const float* mu,
+                            float* ret) {
+  if (!ret) {
+    set_error("ret is NULL");
+    return CPET_ERR_ARG;
+  }
+  cpet_ctx* c = nullptr;
+  int rc = default_ctx(&c);
+  if (rc) return rc;
+  double f[2];
+  const int32_t it = n_iter;
+  rc = cpet_topo_dipole(c, n_dipoles, 1, x_0, &it, step_size, dimensions, x, mu, f, nullptr,
+                        nullptr, 0u);
   if (rc) return rc;
   ret[0] = (float)f[0];
   ret[1] = (float)f[1];

@@ -170,6 +170,96 @@ __device__ __forceinline__ void slice_field(float px, float py, float pz,
   }
 }
 
+// Sources of the field a streamline follows.  A policy gives the FP32
+// slice sum of the propagate kernel and the FP64 field of the epilogue.
+//
+// Charges: one float4 (x, y, z, q) per source (math_module.c:296-333).
+struct ChargeSrc {
+  __device__ static void slice(float px, float py, float pz, const float4* __restrict__ ch,
+                               int t0, int t1, double& ex, double& ey, double& ez) {
+    slice_field(px, py, pz, ch, t0, t1, ex, ey, ez);
+  }
+  __device__ static void field2_f64(const double (&p)[2][3], double (&e)[2][3],
+                                    const float4* __restrict__ ch, int npad) {
+    for (int j = 0; j < npad; ++j) {
+      const float4 c = ch[j];
+      const double cx = c.x, cy = c.y, cz = c.z, q = c.w;
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const double dx = p[k][0] - cx, dy = p[k][1] - cy, dz = p[k][2] - cz;
+        const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        const double ri = rsqrt_f64(r2);
+        const double s = (q * ri) * (ri * ri);
+        e[k][0] = fma(s, dx, e[k][0]);
+        e[k][1] = fma(s, dy, e[k][1]);
+        e[k][2] = fma(s, dz, e[k][2]);
+      }
+    }
+  }
+};
+
+// Point dipoles (calc_field_base_dipole, math_module.c:335-371): two float4
+// per source, (x, y, z, 0) and (mu_x, mu_y, mu_z, 0); the term is
+//   r^-5 (c R - mu r^2),  R = p - x,
+// with the reference's c = 3 mu_x R_x + mu_y R_y + mu_z R_z, or with
+// PHYS the physical c = 3 mu . R.  Padding dipoles (mu = 0 at 1e18) add 0.
+template <bool PHYS>
+struct DipoleSrc {
+  __device__ static float coef(float mx, float my, float mz, float dx, float dy, float dz) {
+    return PHYS ? 3.f * fmaf(mz, dz, fmaf(my, dy, mx * dx)) : fmaf(3.f * mx, dx, fmaf(my, dy, mz * dz));
+  }
+  __device__ static void slice(float px, float py, float pz, const float4* __restrict__ ch,
+                               int t0, int t1, double& ex, double& ey, double& ez) {
+    constexpr int kGroup = 4;  // 4 dipoles = 8 float4 = two 64-byte scalar loads
+    ex = ey = ez = 0.0;
+    for (int t = t0; t < t1; ++t) {
+      float fx = 0.f, fy = 0.f, fz = 0.f;
+      const float4* cp = ch + 2 * t * kTile;
+      for (int g = 0; g < kTile; g += kGroup) {
+        float4 c[2 * kGroup];
+#pragma unroll
+        for (int u = 0; u < 2 * kGroup; ++u) c[u] = cp[2 * g + u];
+#pragma unroll
+        for (int u = 0; u < kGroup; ++u) {
+          const float4 X = c[2 * u], M = c[2 * u + 1];
+          const float dx = px - X.x, dy = py - X.y, dz = pz - X.z;
+          const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+          const float ri = __builtin_amdgcn_rsqf(r2);
+          const float ri2 = ri * ri;
+          const float ri5 = (ri2 * ri2) * ri;
+          const float cc = coef(M.x, M.y, M.z, dx, dy, dz);
+          fx = fmaf(ri5, fmaf(cc, dx, -M.x * r2), fx);
+          fy = fmaf(ri5, fmaf(cc, dy, -M.y * r2), fy);
+          fz = fmaf(ri5, fmaf(cc, dz, -M.z * r2), fz);
+        }
+      }
+      ex += (double)fx;
+      ey += (double)fy;
+      ez += (double)fz;
+    }
+  }
+  __device__ static void field2_f64(const double (&p)[2][3], double (&e)[2][3],
+                                    const float4* __restrict__ ch, int npad) {
+    for (int j = 0; j < npad; ++j) {
+      const float4 X = ch[2 * j], M = ch[2 * j + 1];
+      const double mx = M.x, my = M.y, mz = M.z;
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const double dx = p[k][0] - X.x, dy = p[k][1] - X.y, dz = p[k][2] - X.z;
+        const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        const double ri = rsqrt_f64(r2);
+        const double ri2 = ri * ri;
+        const double ri5 = (ri2 * ri2) * ri;
+        const double cc = PHYS ? 3.0 * fma(mz, dz, fma(my, dy, mx * dx))
+                               : fma(3.0 * mx, dx, fma(my, dy, mz * dz));
+        e[k][0] = fma(ri5, fma(cc, dx, -mx * r2), e[k][0]);
+        e[k][1] = fma(ri5, fma(cc, dy, -my * r2), e[k][1]);
+        e[k][2] = fma(ri5, fma(cc, dz, -mz * r2), e[k][2]);
+      }
+    }
+  }
+};
+
 enum SlotState : int { kActive = 0, kNeed = 1, kDone = 2 };
 
 // One frame's streamline set as the slot loop sees it.  Streamline indices
@@ -193,7 +283,7 @@ struct FrameView {
 // waves apply the same FP64 update.  A finished slot asks the queue for the
 // next streamline (one idle iteration).  first_q >= 0 gives the slot an
 // initial queue position, first_q < 0 starts it empty.
-template <int S>
+template <int S, class Src>
 __device__ __forceinline__ void run_slots(const float4* __restrict__ ch, const FrameView& f,
                                           int first_q, double (*part)[S][3][64],
                                           int (*nsid)[64], int lane, int wave) {
@@ -235,7 +325,7 @@ __device__ __forceinline__ void run_slots(const float4* __restrict__ ch, const F
   while (__ballot(st != kDone) != 0ull) {
     double fx = 0.0, fy = 0.0, fz = 0.0;
     if (st == kActive)
-      slice_field((float)px, (float)py, (float)pz, ch, f.t0, f.t1, fx, fy, fz);
+      Src::slice((float)px, (float)py, (float)pz, ch, f.t0, f.t1, fx, fy, fz);
     part[par][wave][0][lane] = fx;
     part[par][wave][1][lane] = fy;
     part[par][wave][2][lane] = fz;
@@ -273,7 +363,7 @@ __device__ __forceinline__ void run_slots(const float4* __restrict__ ch, const F
 // Single frame.  The charge pointer is a separate __restrict__ parameter so
 // the compiler may read charges through the scalar cache despite the
 // kernel's stores.
-template <int S>
+template <int S, class Src>
 __global__ __launch_bounds__(64 * S, 8) void topo_propagate_kernel(
     const float4* __restrict__ ch, PropArgs a) {
   __shared__ double part[2][S][3][64];
@@ -297,7 +387,7 @@ __global__ __launch_bounds__(64 * S, 8) void topo_propagate_kernel(
   f.counter = a.counter;
   f.t0 = min(wave * tpw, ntiles);
   f.t1 = min(f.t0 + tpw, ntiles);
-  run_slots<S>(ch, f, blockIdx.x * 64 + lane, part, nsid, lane, wave);
+  run_slots<S, Src>(ch, f, blockIdx.x * 64 + lane, part, nsid, lane, wave);
 }
 
 // Many frames in one persistent launch.  Workgroups claim assignments
@@ -356,7 +446,7 @@ __global__ __launch_bounds__(64 * S, 8) void topo_batch_kernel(const float4* __r
     f.counter = a.sl_counter + fr;
     f.t0 = c0 + min(wave * tpw, ntiles);
     f.t1 = c0 + min(wave * tpw + tpw, ntiles);
-    run_slots<S>(ch, f, -1, part, nsid, lane, wave);
+    run_slots<S, ChargeSrc>(ch, f, -1, part, nsid, lane, wave);
   }
 }
 
@@ -377,25 +467,13 @@ struct EpiArgs {
 };
 
 // One Euler step for two points at once, FP64 throughout.
+template <class Src>
 __device__ __forceinline__ void step2_f64(const double (&p)[2][3],
                                           double (&o)[2][3],
                                           const float4* __restrict__ ch,
                                           int npad, double h) {
   double e[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
-  for (int j = 0; j < npad; ++j) {
-    const float4 c = ch[j];
-    const double cx = c.x, cy = c.y, cz = c.z, q = c.w;
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      const double dx = p[k][0] - cx, dy = p[k][1] - cy, dz = p[k][2] - cz;
-      const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-      const double ri = rsqrt_f64(r2);
-      const double s = (q * ri) * (ri * ri);
-      e[k][0] = fma(s, dx, e[k][0]);
-      e[k][1] = fma(s, dy, e[k][1]);
-      e[k][2] = fma(s, dz, e[k][2]);
-    }
-  }
+  Src::field2_f64(p, e, ch, npad);
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     const double ex = kCoulomb * e[k][0], ey = kCoulomb * e[k][1],
@@ -418,6 +496,7 @@ __device__ __forceinline__ double curve_f64(const double (&a)[3],
 }
 
 // Features (and histogram count) of streamline l of one frame.
+template <class Src>
 __device__ __forceinline__ void epilogue_one(const float4* __restrict__ ch, int npad,
                                              const float* starts, const double* fin,
                                              double h, double* feats, int l,
@@ -428,8 +507,8 @@ __device__ __forceinline__ void epilogue_one(const float4* __restrict__ ch, int 
     p[0][i] = (double)starts[3 * l + i];
     p[1][i] = fin[3 * l + i];
   }
-  step2_f64(p, q1, ch, npad, h);
-  step2_f64(q1, q2, ch, npad, h);
+  step2_f64<Src>(p, q1, ch, npad, h);
+  step2_f64<Src>(q1, q2, ch, npad, h);
   double kap[2];
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
@@ -458,6 +537,7 @@ __device__ __forceinline__ void epilogue_one(const float4* __restrict__ ch, int 
   }
 }
 
+template <class Src>
 __global__ __launch_bounds__(256) void topo_epilogue_kernel(
     const float4* __restrict__ ch, EpiArgs a) {
   extern __shared__ unsigned int lh[];
@@ -467,7 +547,7 @@ __global__ __launch_bounds__(256) void topo_epilogue_kernel(
     __syncthreads();
   }
   const int l = blockIdx.x * blockDim.x + threadIdx.x;
-  if (l < a.L) epilogue_one(ch, a.npad, a.starts, a.fin, a.h, a.feats, l, a, lh, a.hist);
+  if (l < a.L) epilogue_one<Src>(ch, a.npad, a.starts, a.fin, a.h, a.feats, l, a, lh, a.hist);
   if (a.hist && a.lds_hist) {
     __syncthreads();
     for (int i = threadIdx.x; i < nbins; i += blockDim.x)
@@ -494,7 +574,7 @@ __global__ __launch_bounds__(256) void topo_batch_epilogue_kernel(
   const int l = b * blockDim.x + threadIdx.x;
   unsigned int* hist = a.hist ? a.hist + (size_t)fr * nbins : nullptr;
   if (l < L)
-    epilogue_one(ch + chg_off[fr], chg_off[fr + 1] - chg_off[fr], a.starts + 3LL * s0,
+    epilogue_one<ChargeSrc>(ch + chg_off[fr], chg_off[fr + 1] - chg_off[fr], a.starts + 3LL * s0,
                  a.fin + 3LL * s0, hdims[4 * fr], a.feats + 2LL * s0, l, a, lh, hist);
   if (a.hist && a.lds_hist) {
     __syncthreads();
@@ -503,10 +583,22 @@ __global__ __launch_bounds__(256) void topo_batch_epilogue_kernel(
   }
 }
 
-template <int S>
+template <int S, class Src>
 hipError_t launch_prop(hipStream_t st, int G, const float4* ch, const PropArgs& a) {
-  hipLaunchKernelGGL((topo_propagate_kernel<S>), dim3(G), dim3(64 * S), 0, st, ch, a);
+  hipLaunchKernelGGL((topo_propagate_kernel<S, Src>), dim3(G), dim3(64 * S), 0, st, ch, a);
   return hipGetLastError();
+}
+
+template <class Src>
+hipError_t launch_prop_s(int S, hipStream_t st, int G, const float4* ch, const PropArgs& a) {
+  switch (S) {
+    case 1: return launch_prop<1, Src>(st, G, ch, a);
+    case 2: return launch_prop<2, Src>(st, G, ch, a);
+    case 4: return launch_prop<4, Src>(st, G, ch, a);
+    case 8: return launch_prop<8, Src>(st, G, ch, a);
+    case 16: return launch_prop<16, Src>(st, G, ch, a);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 }  // namespace
@@ -515,7 +607,7 @@ int topo_run(cpet_ctx* ctx, int N, int L, const float* starts,
              const int32_t* n_iter, float h, const float* dims, const float4* ch,
              int npad, double* feats, int32_t* steps, int32_t* exited, int nb_d,
              const double* edges_d, int nb_c, const double* edges_c,
-             uint32_t* hist, uint32_t flags) {
+             uint32_t* hist, uint32_t flags, int src) {
   (void)N;
   const bool dev = (flags & CPET_F_DEVICE) != 0;
   hipStream_t st = ctx->stream;
@@ -601,17 +693,13 @@ int topo_run(cpet_ctx* ctx, int N, int L, const float* starts,
   pa.steps = d_steps;
   pa.exited = d_exited;
   pa.counter = d_counter;
-  hipError_t e;
-  switch (S) {
-    case 1: e = launch_prop<1>(st, G, ch, pa); break;
-    case 2: e = launch_prop<2>(st, G, ch, pa); break;
-    case 4: e = launch_prop<4>(st, G, ch, pa); break;
-    case 8: e = launch_prop<8>(st, G, ch, pa); break;
-    case 16: e = launch_prop<16>(st, G, ch, pa); break;
-    default:
-      set_error("unsupported waves per workgroup %d (1,2,4,8,16)", S);
-      return CPET_ERR_ARG;
+  if (S != 1 && S != 2 && S != 4 && S != 8 && S != 16) {
+    set_error("unsupported waves per workgroup %d (1,2,4,8,16)", S);
+    return CPET_ERR_ARG;
   }
+  hipError_t e = src == kSrcCharge       ? launch_prop_s<ChargeSrc>(S, st, G, ch, pa)
+                 : src == kSrcDipoleRef  ? launch_prop_s<DipoleSrc<false>>(S, st, G, ch, pa)
+                                         : launch_prop_s<DipoleSrc<true>>(S, st, G, ch, pa);
   if (e != hipSuccess) {
     set_error("topo propagate launch failed: %s", hipGetErrorString(e));
     return CPET_ERR_HIP;
@@ -636,7 +724,15 @@ int topo_run(cpet_ctx* ctx, int N, int L, const float* starts,
     hipLaunchKernelGGL(fill_u32_kernel, dim3((nbins + 255) / 256), dim3(256), 0, st,
                        d_hist, (long long)nbins, 0u);
   const size_t lds = ea.lds_hist ? (size_t)nbins * sizeof(unsigned int) : 0;
-  hipLaunchKernelGGL(topo_epilogue_kernel, dim3((L + 255) / 256), dim3(256), lds, st, ch, ea);
+  if (src == kSrcCharge)
+    hipLaunchKernelGGL(topo_epilogue_kernel<ChargeSrc>, dim3((L + 255) / 256), dim3(256), lds,
+                       st, ch, ea);
+  else if (src == kSrcDipoleRef)
+    hipLaunchKernelGGL(topo_epilogue_kernel<DipoleSrc<false>>, dim3((L + 255) / 256), dim3(256),
+                       lds, st, ch, ea);
+  else
+    hipLaunchKernelGGL(topo_epilogue_kernel<DipoleSrc<true>>, dim3((L + 255) / 256), dim3(256),
+                       lds, st, ch, ea);
   CPET_HIP_TRY(hipGetLastError());
   if ((rc = prof_mark(ctx, 3))) return rc;
 

@@ -198,3 +198,44 @@ def topo(starts, n_iter, h: float, dims, x, q, *, edges=None,
         nb_c, _ptr(ec) if ec is not None else None,
         _ptr(hist) if hist is not None else None, flags), ctx.lib)
     return TopoResult(feats, steps, exited, hist)
+
+
+def topo_dipole(starts, n_iter, h: float, dims, x, mu, *, physical: bool = False,
+                ctx: Optional[nat.Context] = None) -> TopoResult:
+    """Streamlines in the field of point dipoles x (N,3) with moments mu
+    (N,3): thread_operation_dipole semantics (CPET/utils/math_module.c:
+    593-659) with the reference's pair term, or the physical one."""
+    ctx = ctx or nat.default_context()
+    flags = nat.F_DIPOLE_PHYSICAL if physical else 0
+    h = float(np.float32(h))
+    if _is_torch(dims):
+        dims = dims.detach().cpu().numpy()
+    dims = _host_f32(dims).ravel()
+    if _is_torch(x):
+        import torch
+
+        dev = _torch_dev_args(ctx, starts, n_iter, x, mu)
+        L = starts.shape[0]
+        feats = torch.empty((L, 2), dtype=torch.float64, device=dev)
+        steps = torch.empty((L,), dtype=torch.int32, device=dev)
+        exited = torch.empty((L,), dtype=torch.int32, device=dev)
+        flags |= nat.F_DEVICE | nat.F_NO_SYNC
+        N = x.shape[0]
+    else:
+        starts = _host_f32(starts, 3)
+        n_iter = np.ascontiguousarray(n_iter, dtype=np.int32).ravel()
+        x = _host_f32(x, 3)
+        mu = _host_f32(mu, 3)
+        L = starts.shape[0]
+        feats = np.empty((L, 2), dtype=np.float64)
+        steps = np.empty((L,), dtype=np.int32)
+        exited = np.empty((L,), dtype=np.int32)
+        N = x.shape[0]
+    if n_iter.shape[0] != L:
+        raise ValueError("n_iter must have one entry per start point")
+    if tuple(mu.shape) != (N, 3):
+        raise ValueError("mu must be (N, 3) like x")
+    nat.check(ctx.lib.cpet_topo_dipole(
+        ctx.handle, N, L, _ptr(starts), _ptr(n_iter), ctypes.c_float(h), _ptr(dims), _ptr(x),
+        _ptr(mu), _ptr(feats), _ptr(steps), _ptr(exited), flags), ctx.lib)
+    return TopoResult(feats, steps, exited, None)

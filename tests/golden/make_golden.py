@@ -6,6 +6,7 @@ them (CPET/utils/c_ops.py:250-341 and CPET/utils/calculator.py:430-475).
 Run in a checkout where the reference source exists:
     make -C oracle ref && python tests/golden/make_golden.py
     python tests/golden/make_golden.py pdb      # only the text fixtures
+    python tests/golden/make_golden.py dipole   # only ref_topo_dipole.npz
 
 Each .npz holds the seeded inputs and the reference's FP32 outputs only.
 examples_point_field/ holds the reference's point-field example: its two
@@ -91,7 +92,28 @@ def pdb_rows():
     (OUT / "ref_pdb_rows.pdb").write_text("".join(keep))
 
 
+def topo_dipole(ref):
+    """thread_operation_dipole of the reference C on seeded dipoles."""
+    rng = np.random.default_rng(21)
+    N, L = 80, 160
+    d = rng.standard_normal((N, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    x = (d * rng.uniform(2.0, 7.0, (N, 1))).astype(np.float32)
+    mu = rng.normal(0.0, 0.3, (N, 3)).astype(np.float32)
+    dims = np.array([0.5, 0.45, 0.55], np.float32)
+    starts = rng.uniform(-0.4, 0.4, (L, 3)).astype(np.float32)
+    n_iter = rng.integers(1, 150, L).astype(np.int32)
+    h = np.float32(0.01)
+    feats = np.stack([ref.thread_operation_dipole(starts[i], n_iter[i], h, dims, x, mu)
+                      for i in range(L)])
+    np.savez_compressed(OUT / "ref_topo_dipole.npz", x=x, mu=mu, dims=dims, starts=starts,
+                        n_iter=n_iter, h=h, feats=feats)
+
+
 if __name__ == "__main__":
+    if sys.argv[1:] == ["dipole"]:  # only the dipole vectors
+        topo_dipole(Ref())
+        sys.exit(0)
     pdb_rows()
     point_field_examples()
     if sys.argv[1:] == ["pdb"]:  # text fixtures only
@@ -100,5 +122,6 @@ if __name__ == "__main__":
     volume(r)
     volume_floor(r)
     topo(r)
+    topo_dipole(r)
     for p in sorted(OUT.glob("*.npz")):
         print(p.name, p.stat().st_size)

@@ -116,3 +116,86 @@ def test_one_step_and_zero_steps(oracle_mod):
     assert list(steps) == [1, 0] and list(exited) == [0, 0]
     np.testing.assert_allclose(feats[0, 0], float(np.float32(0.01)), rtol=1e-12)
     assert feats[1, 0] == 0.0
+
+
+def _dipoles(seed, N=60, R=6.0):
+    rng = np.random.default_rng(seed)
+    d = rng.standard_normal((N, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    x = (d * rng.uniform(2.0, R, (N, 1))).astype(np.float32)
+    mu = rng.normal(0.0, 0.3, (N, 3)).astype(np.float32)
+    return x, mu
+
+
+def test_dipole_oracle_matches_reference_c(oracle_mod):
+    """FP64 dipole walk vs the reference's own thread_operation_dipole (FP32,
+    -ffast-math): steps and stop kind on >= 90 % of starts, distances there
+    within 1e-3 (box-exit ties and FP32 drift account for the rest)."""
+    if not oracle_mod.Ref.available():
+        pytest.skip("reference C not built here")
+    ref = oracle_mod.Ref()
+    x, mu = _dipoles(3)
+    rng = np.random.default_rng(4)
+    dims = np.array([0.5, 0.5, 0.5], np.float32)
+    starts = rng.uniform(-0.45, 0.45, (60, 3)).astype(np.float32)
+    n_iter = rng.integers(1, 120, 60).astype(np.int32)
+    feats, steps, exited = oracle_mod.topo_dipole_f64(starts, n_iter, 0.01, dims, x, mu)
+    want = np.stack([ref.thread_operation_dipole(starts[i], n_iter[i], 0.01, dims, x, mu)
+                     for i in range(60)])
+    close = np.abs(feats[:, 0] - want[:, 0]) <= 1e-3 * np.maximum(want[:, 0], 1e-2)
+    assert close.mean() >= 0.9, close.mean()
+    # curvature is an FP32 finite difference in the reference: order of magnitude
+    ok = close & (want[:, 1] > 0)
+    assert np.median(np.abs(np.log10(feats[ok, 1] / want[ok, 1]))) < 0.5
+
+
+def test_dipole_on_axis_exact(oracle_mod):
+    """A dipole mu x-hat at the origin: on the x axis both field forms give
+    E = 2 k mu / r^3 x-hat, so a start (a, 0, 0) walks straight out along x
+    and stops on the first step past the box face, curvature 0."""
+    dims = np.array([0.6, 0.5, 0.4], np.float32)
+    h = 0.01
+    x = np.array([[-3.0, 0.0, 0.0]], np.float32)
+    mu = np.array([[0.7, 0.0, 0.0]], np.float32)
+    a = np.array([[-0.31, 0, 0], [0.12, 0, 0], [0.55, 0, 0]], np.float32)
+    n_iter = np.array([500, 500, 3], np.int32)
+    for physical in (False, True):
+        feats, steps, exited = oracle_mod.topo_dipole_f64(a, n_iter, h, dims, x, mu,
+                                                          physical=physical)
+        exp_steps, clean, exp_exit = _radial_expect(a, x[0].astype(float), 1.0, dims, h, n_iter)
+        assert np.all(steps[clean] == exp_steps[clean]) and np.all(exited[clean] == exp_exit[clean])
+        np.testing.assert_allclose(feats[clean, 0], steps[clean] * np.float64(np.float32(h)),
+                                   rtol=1e-12)
+        assert np.all(feats[:, 1] < 1e-9)
+
+
+def test_dipole_forms_agree_for_x_moments(oracle_mod):
+    """With every moment along x the reference's coefficient 3 mu_x R_x +
+    mu_y R_y + mu_z R_z equals the physical 3 mu . R, so both walks agree;
+    with general moments they differ."""
+    x, mu = _dipoles(8)
+    rng = np.random.default_rng(9)
+    dims = np.array([0.5, 0.5, 0.5], np.float32)
+    starts = rng.uniform(-0.4, 0.4, (30, 3)).astype(np.float32)
+    n_iter = np.full(30, 80, np.int32)
+    mux = mu.copy()
+    mux[:, 1:] = 0.0
+    a = oracle_mod.topo_dipole_f64(starts, n_iter, 0.01, dims, x, mux)
+    b = oracle_mod.topo_dipole_f64(starts, n_iter, 0.01, dims, x, mux, physical=True)
+    np.testing.assert_array_equal(a[1], b[1])
+    np.testing.assert_allclose(a[0], b[0], rtol=1e-12, atol=1e-15)
+    c = oracle_mod.topo_dipole_f64(starts, n_iter, 0.01, dims, x, mu)
+    d = oracle_mod.topo_dipole_f64(starts, n_iter, 0.01, dims, x, mu, physical=True)
+    assert np.abs(c[0] - d[0]).max() > 1e-3
+
+
+def test_dipole_oracle_matches_golden(oracle_mod):
+    """The FP64 dipole walk against the reference C's own outputs kept in
+    tests/golden/ref_topo_dipole.npz (FP32 -ffast-math): distance within
+    1e-3 on >= 90 % of the 160 starts."""
+    g = np.load(GOLDEN / "ref_topo_dipole.npz")
+    feats, _, _ = oracle_mod.topo_dipole_f64(g["starts"], g["n_iter"], float(g["h"]), g["dims"],
+                                             g["x"], g["mu"])
+    want = g["feats"].astype(float)
+    close = np.abs(feats[:, 0] - want[:, 0]) <= 1e-3 * np.maximum(want[:, 0], 1e-2)
+    assert close.mean() >= 0.9, close.mean()
