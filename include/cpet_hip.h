@@ -115,9 +115,13 @@ int cpet_device_count(int *n);
  * No counterpart in the reference (which has no device state). */
 int cpet_ctx_create(int device, cpet_ctx **out);
 int cpet_ctx_destroy(cpet_ctx *ctx);
-/* Use the caller's stream (e.g. torch.cuda.current_stream().cuda_stream)
- * instead of the context's own; NULL restores the own stream. */
+/* Queue the context's work on the caller's stream (e.g.
+ * torch.cuda.current_stream().cuda_stream); NULL is the null (legacy
+ * default) stream, the handle torch reports for its default stream.  With
+ * device pointers the caller's allocator sees the work in its own stream
+ * order.  cpet_ctx_use_own_stream returns to the context's private stream. */
 int cpet_ctx_set_stream(cpet_ctx *ctx, void *hip_stream);
+int cpet_ctx_use_own_stream(cpet_ctx *ctx);
 int cpet_ctx_get_stream(cpet_ctx *ctx, void **hip_stream);
 int cpet_ctx_sync(cpet_ctx *ctx);
 

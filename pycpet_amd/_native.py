@@ -69,6 +69,7 @@ SIGNATURES = {
     "cpet_ctx_destroy": [c_void_p],
     "cpet_ctx_set_stream": [c_void_p, c_void_p],
     "cpet_ctx_get_stream": [c_void_p, P],
+    "cpet_ctx_use_own_stream": [c_void_p],
     "cpet_ctx_sync": [c_void_p],
     "cpet_field_points": [c_void_p, c_int, c_int, P, P, P, P, P, c_int, c_uint32],
     "cpet_field_grid": [c_void_p, c_int, c_int, c_int, P, P, P, c_int, P, P, P, P, c_int, c_uint32],
@@ -175,7 +176,12 @@ class Context:
         self.device = device
 
     def set_stream(self, stream_handle: int | None) -> None:
+        """Queue work on this HIP stream handle; 0 / None is the null stream
+        (torch's default stream)."""
         check(self.lib.cpet_ctx_set_stream(self.handle, stream_handle or None), self.lib)
+
+    def use_own_stream(self) -> None:
+        check(self.lib.cpet_ctx_use_own_stream(self.handle), self.lib)
 
     def stream(self) -> int:
         s = ctypes.c_void_p()

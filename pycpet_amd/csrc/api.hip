@@ -360,7 +360,7 @@ int cpet_ctx_destroy(cpet_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->own_stream && ctx->own)
       (void)hipStreamSynchronize(ctx->own);
-    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream);  // the caller's stream, or the null stream
     for (auto& b : ctx->bufs)
       if (b.ptr) (void)hipFree(b.ptr);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
@@ -382,7 +382,20 @@ int cpet_ctx_set_stream(cpet_ctx* ctx, void* hip_stream) {
     return CPET_ERR_ARG;
   }
   std::lock_guard<std::mutex> lk(ctx->mu);
-  ctx->stream = hip_stream ? (hipStream_t)hip_stream : ctx->own;
+  // NULL is the null (legacy default) stream, which is what torch reports
+  // for its default stream: work queued there is ordered with torch's, so
+  // torch may free and reuse its tensors as soon as its stream allows.
+  ctx->stream = (hipStream_t)hip_stream;
+  return CPET_OK;
+}
+
+int cpet_ctx_use_own_stream(cpet_ctx* ctx) {
+  if (!ctx) {
+    set_error("ctx is NULL");
+    return CPET_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->stream = ctx->own;
   return CPET_OK;
 }
 
