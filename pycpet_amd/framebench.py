@@ -12,8 +12,8 @@ per frame), inputs resident in HBM, timed three ways:
   per-frame one launch per frame with the tuned single-frame kernels;
   batched   all volume frames in one launch and all streamline frames in one
             persistent launch, on two streams;
-  batched_graph  the same two batched steps captured into HIP graphs
-            (pycpet_amd.graphs) and replayed on the two streams.
+  batched_graph  the same two batched launches captured as parallel
+            branches of one HIP graph (pycpet_amd.graphs) and replayed.
 serve: Poisson arrivals (pycpet_amd.serve.poisson_requests) served with
 continuous batching vs one request at a time (stopped after --plain-max-s,
 unserved requests counted with their latency lower bound); p50/p99 latency
@@ -98,14 +98,12 @@ def run_batch(n_frames: int, topo_starts: int, seed: int = 3) -> dict:
     # the same batched step captured into two HIP graphs (one per stream)
     from . import graphs as G
 
-    gv = G.CapturedStep(lambda c: B.batch_volume(vol, ctx=c), tuning={4: kv})
-    gt = G.CapturedStep(lambda c: B.batch_topo(top, ctx=c), tuning={3: kt})
+    # both batched launches as parallel branches of one HIP graph
+    gs = G.CapturedStep([lambda c: B.batch_volume(vol, ctx=c), lambda c: B.batch_topo(top, ctx=c)],
+                        tuning=[{4: kv}, {3: kt}])
 
     def batched_graph():
-        with torch.cuda.stream(s1):
-            gv.replay()
-        with torch.cuda.stream(s2):
-            gt.replay()
+        gs.replay()
         torch.cuda.synchronize()
 
     timings = {}
@@ -141,8 +139,9 @@ def run_batch(n_frames: int, topo_starts: int, seed: int = 3) -> dict:
 
     out["eager_rerun_maxdiff"] = max(maxdiff(first["vol"], results["vol"], lambda r: r[0]),
                                      maxdiff(first["top"], results["top"], lambda r: r.feats))
-    out["graph_vs_eager_maxdiff"] = max(maxdiff(gv.outputs, results["vol"], lambda r: r[0]),
-                                        maxdiff(gt.outputs, results["top"], lambda r: r.feats))
+    gv_out, gt_out = gs.outputs
+    out["graph_vs_eager_maxdiff"] = max(maxdiff(gv_out, results["vol"], lambda r: r[0]),
+                                        maxdiff(gt_out, results["top"], lambda r: r.feats))
     out["batched_vs_plain"] = timings["plain"] / timings["batched"]
     out["graph_vs_plain"] = timings["plain"] / timings["batched_graph"]
     out["batched_vs_per_frame"] = timings["per_frame"] / timings["batched"]

@@ -6,7 +6,8 @@ on torch device tensors of fixed shapes.  `CapturedStep(fn)` gives it a
 context of its own, runs it eagerly (scratch buffers grow, the batch tables
 are uploaded), then captures one run into a graph with torch.cuda.graph.
 `replay()` re-launches every kernel of the step on the current contents of
-the input tensors and rewrites the same output tensors.
+the input tensors and rewrites the same output tensors.  Several step
+functions become parallel branches of one graph.
 
 Why this is safe to capture: with device pointers and CPET_F_NO_SYNC the
 library neither waits on the stream nor reads it back; scratch buffers only
@@ -27,26 +28,52 @@ from . import _native as nat
 
 
 class CapturedStep:
-    def __init__(self, fn: Callable[[nat.Context], Any], device: int = 0, warmup: int = 1,
-                 tuning: Optional[Dict[int, Tuple[int, int]]] = None):
+    """fn: a step function of a context, or a list of them.  A list is
+    captured as parallel branches of one graph (each branch on its own
+    context and side stream, forked from and joined back to the capture
+    stream), so one replay runs e.g. the batched volume and streamline
+    launches concurrently.  tuning: {kind: (a, b)} for every branch, or a
+    list with one dict per branch."""
+
+    def __init__(self, fn, device: int = 0, warmup: int = 1, tuning=None):
         import torch
 
         self.torch = torch
-        self.ctx = nat.Context(device)
-        self.ctx.set_profiling(False)
-        for kind, (a, b) in (tuning or {}).items():  # launch shapes are fixed at capture
-            self.ctx.set_tuning(kind, a, b)
-        side = torch.cuda.Stream(device=device)
-        side.wait_stream(torch.cuda.current_stream(device))
-        with torch.cuda.stream(side):
-            for _ in range(max(1, warmup)):
-                fn(self.ctx)
-        torch.cuda.current_stream(device).wait_stream(side)
+        fns = list(fn) if isinstance(fn, (list, tuple)) else [fn]
+        tunings = list(tuning) if isinstance(tuning, (list, tuple)) else [tuning] * len(fns)
+        if len(tunings) != len(fns):
+            raise ValueError("one tuning dict per branch")
+        self.ctxs = []
+        for t in tunings:
+            c = nat.Context(device)
+            c.set_profiling(False)
+            for kind, (a, b) in (t or {}).items():  # launch shapes are fixed at capture
+                c.set_tuning(kind, a, b)
+            self.ctxs.append(c)
+        self.ctx = self.ctxs[0]
+        main = torch.cuda.current_stream(device)
+        sides = [torch.cuda.Stream(device=device) for _ in fns]
+        for s in sides:
+            s.wait_stream(main)
+        for f, c, s in zip(fns, self.ctxs, sides):
+            with torch.cuda.stream(s):
+                for _ in range(max(1, warmup)):
+                    f(c)
+        for s in sides:
+            main.wait_stream(s)
         torch.cuda.synchronize(device)
         self.graph = torch.cuda.CUDAGraph()
+        outs = []
         with torch.cuda.graph(self.graph):
-            self.outputs = fn(self.ctx)
+            cap = torch.cuda.current_stream(device)
+            for f, c, s in zip(fns, self.ctxs, sides):
+                s.wait_stream(cap)
+                with torch.cuda.stream(s):
+                    outs.append(f(c))
+            for s in sides:
+                cap.wait_stream(s)
         torch.cuda.synchronize(device)
+        self.outputs = outs if isinstance(fn, (list, tuple)) else outs[0]
 
     def replay(self):
         """Runs the captured step on torch's current stream; returns the
@@ -57,4 +84,5 @@ class CapturedStep:
     def close(self) -> None:
         self.graph = None
         self.outputs = None
-        self.ctx.close()
+        for c in self.ctxs:
+            c.close()

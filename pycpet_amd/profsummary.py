@@ -24,15 +24,20 @@ def kernel_rows(db: str):
 
 
 def pmc_rows(db: str):
+    """(kernel, counter, dispatches, mean value per dispatch) rows."""
     con = sqlite3.connect(db)
     cur = con.cursor()
-    try:
-        rows = cur.execute(
-            "select kernel_name, counter_name, count(*), avg(value) from pmc_events "
-            "group by kernel_name, counter_name order by kernel_name").fetchall()
-    except sqlite3.Error:
-        cols = [r[1] for r in cur.execute("pragma table_info(pmc_events)")]
+    cols = [r[1] for r in cur.execute("pragma table_info(pmc_events)")]
+    if {"name", "counter_name", "counter_value"} <= set(cols):
+        q = ("select name, counter_name, count(*), avg(counter_value) from pmc_events "
+             "group by name, counter_name order by name")
+    elif {"kernel_name", "counter_name", "value"} <= set(cols):
+        q = ("select kernel_name, counter_name, count(*), avg(value) from pmc_events "
+             "group by kernel_name, counter_name order by kernel_name")
+    else:
+        con.close()
         raise SystemExit(f"unexpected pmc_events layout: {cols}")
+    rows = cur.execute(q).fetchall()
     con.close()
     return rows
 

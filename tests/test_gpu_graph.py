@@ -98,3 +98,31 @@ def test_captured_batched_step(gpu_ctx):
         for r, r2 in zip(outs_t, et):
             assert torch.equal(r.feats, r2.feats) and torch.equal(r.steps, r2.steps)
     step.close()
+
+
+def test_parallel_branches(gpu_ctx):
+    """A list of step functions becomes parallel branches of one graph; each
+    branch's outputs equal its eager run."""
+    import torch
+
+    from pycpet_amd import batch, graphs, workloads as W
+
+    frames = W.md_frames(4, seed=12, n_min=500, n_max=2000, edges=(9,),
+                         topo_starts_per_frame=500, cap=120)
+    vol = [{"x": _dev(f["x"]), "q": _dev(f["q"]), "axes": tuple(_dev(a) for a in f["axes"])}
+           for f in frames if f["kind"] == "volume"]
+    top = [{"x": _dev(f["x"]), "q": _dev(f["q"]), "starts": _dev(f["starts"]),
+            "n_iter": _dev(f["n_iter"]), "h": f["h"], "dims": f["dims"]}
+           for f in frames if f["kind"] == "topo"]
+    step = graphs.CapturedStep([lambda c: batch.batch_volume(vol, ctx=c),
+                                lambda c: batch.batch_topo(top, ctx=c)])
+    for k in range(3):
+        ov, ot = step.replay()
+        torch.cuda.synchronize()
+        ev = batch.batch_volume(vol, ctx=gpu_ctx)
+        et = batch.batch_topo(top, ctx=gpu_ctx)
+        torch.cuda.synchronize()
+        print(f"replay {k}", flush=True)
+        assert all(torch.equal(a[0], b[0]) for a, b in zip(ov, ev))
+        assert all(torch.equal(a.feats, b.feats) for a, b in zip(ot, et))
+    step.close()
