@@ -672,7 +672,7 @@ int topo_run(cpet_ctx* ctx, int N, int L, const float* starts,
   // 2. propagate
   int S = ctx->tune[2][0];
   int G = ctx->tune[2][1];
-  if (S <= 0) S = 8;
+  if (S <= 0) S = 16;  // 16 waves x 64 slots: fastest measured (S 4/8/16, G 256..2048)
   const int gmax = (L + 63) / 64;
   if (G <= 0) G = std::min(gmax, std::max(1, ctx->num_cus * 2));
   G = std::min(G, gmax);
