@@ -856,7 +856,7 @@ int topo_batch_run(cpet_ctx* ctx, int F, const int32_t* n_starts, const float* s
     }
   }
 
-  int S = ctx->tune[3][0] > 0 ? ctx->tune[3][0] : 8;
+  int S = ctx->tune[3][0] > 0 ? ctx->tune[3][0] : 16;  // as the single-frame kernel: same results
   int G = ctx->tune[3][1] > 0 ? ctx->tune[3][1] : ctx->num_cus * 2;
   G = std::max(1, std::min<int>(G, (int)assign.size()));
   BatchTopoArgs ba;
