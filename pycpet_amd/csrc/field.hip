@@ -77,7 +77,7 @@ __device__ __forceinline__ double rsqrt_refine(float seed, double x) {
 
 // Relative error target of the auto-mode check (see above): points whose
 // estimate u * A / |E| exceeds it are recomputed in FP64.
-constexpr double kAutoTol = 4e-6;
+constexpr double kAutoTol = 6e-6;
 constexpr double kUnit32 = 5.9604644775390625e-08;  // 2^-24
 
 // Slots of the per-point FP64 accumulator: E x, y, z, phi, A (auto E only).
