@@ -182,8 +182,14 @@ struct ChargeSrc {
   }
   __device__ static void field2_f64(const double (&p)[2][3], double (&e)[2][3],
                                     const float4* __restrict__ ch, int npad) {
-    for (int j = 0; j < npad; ++j) {
-      const float4 c = ch[j];
+    constexpr int kGroup = 8;  // one scalar-load wait per 8 charges (npad % 64 == 0)
+    for (int g = 0; g < npad; g += kGroup) {
+      float4 cg[kGroup];
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u) cg[u] = ch[g + u];
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u) {
+      const float4 c = cg[u];
       const double cx = c.x, cy = c.y, cz = c.z, q = c.w;
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
@@ -194,6 +200,7 @@ struct ChargeSrc {
         e[k][0] = fma(s, dx, e[k][0]);
         e[k][1] = fma(s, dy, e[k][1]);
         e[k][2] = fma(s, dz, e[k][2]);
+      }
       }
     }
   }
@@ -241,8 +248,14 @@ struct DipoleSrc {
   }
   __device__ static void field2_f64(const double (&p)[2][3], double (&e)[2][3],
                                     const float4* __restrict__ ch, int npad) {
-    for (int j = 0; j < npad; ++j) {
-      const float4 X = ch[2 * j], M = ch[2 * j + 1];
+    constexpr int kGroup = 4;  // 4 dipoles = 8 float4 per scalar-load wait
+    for (int g = 0; g < npad; g += kGroup) {
+      float4 cg[2 * kGroup];
+#pragma unroll
+      for (int u = 0; u < 2 * kGroup; ++u) cg[u] = ch[2 * g + u];
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u) {
+      const float4 X = cg[2 * u], M = cg[2 * u + 1];
       const double mx = M.x, my = M.y, mz = M.z;
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
@@ -256,6 +269,7 @@ struct DipoleSrc {
         e[k][0] = fma(ri5, fma(cc, dx, -mx * r2), e[k][0]);
         e[k][1] = fma(ri5, fma(cc, dy, -my * r2), e[k][1]);
         e[k][2] = fma(ri5, fma(cc, dz, -mz * r2), e[k][2]);
+      }
       }
     }
   }
