@@ -37,13 +37,22 @@ namespace {
 constexpr int kNumBuckets = 4096;
 
 // 1/sqrt(x) from the FP32 hardware estimate refined by one FP64 Newton step
-// (relative error ~1e-7 -> ~1e-14), cheaper than the FP64 estimate plus two
-// steps above.  Clamping y*y keeps x == 0 at +inf (as 1/sqrt(0)).  x must be
-// below FLT_MAX (padding sources at 1e18 give 3e36).
+// (relative error ~1e-7 -> ~1e-14), cheaper than an FP64 square root and
+// divide.  Clamping y*y keeps x == 0 at +inf (as 1/sqrt(0)).  x must lie in
+// the FP32 normal range (a pair's r^2; padding sources at 1e18 give 3e36).
 __device__ __forceinline__ double rsqrt_mixed(double x) {
   const double y = (double)__builtin_amdgcn_rsqf((float)x);
   const double e = fma(-x, fmin(y * y, 1.7976931348623157e308), 1.0);
   return fma(0.5 * y, e, y);
+}
+
+// 1/sqrt(x) over the whole FP64 range: the FP64 hardware estimate and two
+// Newton steps.  x == 0 and x == inf give NaN, as E/|E| does for E = 0 or
+// an infinite E.
+__device__ __forceinline__ double rsqrt_f64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  y = fma(0.5 * y, fma(-x * y, y, 1.0), y);
+  return fma(0.5 * y, fma(-x * y, y, 1.0), y);
 }
 
 __global__ __launch_bounds__(256) void fill_u32_kernel(unsigned int* p,
@@ -143,27 +152,54 @@ __device__ __forceinline__ void slice_field(float px, float py, float pz,
                                             const float4* __restrict__ ch,
                                             int t0, int t1, double& ex,
                                             double& ey, double& ez) {
-  // Charges are fetched 8 at a time (two 64-byte scalar loads issued
-  // together), so the scalar-cache latency is paid once per 8 charges.
-  constexpr int kGroup = 8;
+  // Charges stream through two SGPR buffers of 4 (one 64-byte scalar load
+  // each).  The load of the next group is issued only after the current
+  // group has arrived (an explicit lgkmcnt(0) wait: scalar loads return out
+  // of order, so any later wait covers every load in flight), and a
+  // scheduling barrier keeps it ahead of the current group's 52 VALU
+  // instructions, which then hide its latency.  The stream runs on across
+  // tile boundaries; the last prefetch is clamped to the wave's last group,
+  // so no load leaves the wave's charge range.
+  constexpr int kGroup = 4;
+  constexpr int kWaitLgkm0 = 0xc07f;  // s_waitcnt lgkmcnt(0), other counters untouched
   ex = ey = ez = 0.0;
+  if (t0 >= t1) return;
+  const int last = t1 * kTile - kGroup;  // index of the wave's last group
+  int nx = t0 * kTile;
+  float4 a[kGroup], b[kGroup];
+#pragma unroll
+  for (int u = 0; u < kGroup; ++u) a[u] = ch[nx + u];
+  auto pairs = [&](const float4 (&c)[kGroup], float& fx, float& fy, float& fz) {
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) {
+      const float dx = px - c[u].x, dy = py - c[u].y, dz = pz - c[u].z;
+      const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+      const float ri = __builtin_amdgcn_rsqf(r2);
+      const float s = (c[u].w * ri) * (ri * ri);
+      fx = fmaf(s, dx, fx);
+      fy = fmaf(s, dy, fy);
+      fz = fmaf(s, dz, fz);
+    }
+  };
   for (int t = t0; t < t1; ++t) {
     float fx = 0.f, fy = 0.f, fz = 0.f;
-    const float4* cp = ch + t * kTile;
-    for (int g = 0; g < kTile; g += kGroup) {
-      float4 c[kGroup];
 #pragma unroll
-      for (int u = 0; u < kGroup; ++u) c[u] = cp[g + u];
+    for (int g = 0; g < kTile; g += 2 * kGroup) {
+      __builtin_amdgcn_sched_barrier(0);
+      __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+      nx += kGroup;
 #pragma unroll
-      for (int u = 0; u < kGroup; ++u) {
-        const float dx = px - c[u].x, dy = py - c[u].y, dz = pz - c[u].z;
-        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-        const float ri = __builtin_amdgcn_rsqf(r2);
-        const float s = (c[u].w * ri) * (ri * ri);
-        fx = fmaf(s, dx, fx);
-        fy = fmaf(s, dy, fy);
-        fz = fmaf(s, dz, fz);
-      }
+      for (int u = 0; u < kGroup; ++u) b[u] = ch[nx + u];
+      __builtin_amdgcn_sched_barrier(0);
+      pairs(a, fx, fy, fz);
+      __builtin_amdgcn_sched_barrier(0);
+      __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+      nx += kGroup;
+      const int pf = min(nx, last);
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u) a[u] = ch[pf + u];
+      __builtin_amdgcn_sched_barrier(0);
+      pairs(b, fx, fy, fz);
     }
     ex += (double)fx;
     ey += (double)fy;
@@ -358,10 +394,12 @@ __device__ __forceinline__ void run_slots(const float4* __restrict__ ch, const F
       ex *= kCoulomb;
       ey *= kCoulomb;
       ez *= kCoulomb;
-      const double n = sqrt(ex * ex + ey * ey + ez * ez);
-      px = px + f.h * ex / n;
-      py = py + f.h * ey / n;
-      pz = pz + f.h * ez / n;
+      // x += h E/|E|, with one reciprocal square root instead of an FP64
+      // square root and three divides.
+      const double hn = f.h * rsqrt_f64(fma(ez, ez, fma(ey, ey, ex * ex)));
+      px = fma(hn, ex, px);
+      py = fma(hn, ey, py);
+      pz = fma(hn, ez, pz);
       ++step;
       const int out = (px < -f.d0 || px > f.d0 || py < -f.d1 || py > f.d1 ||
                        pz < -f.d2 || pz > f.d2)
@@ -689,7 +727,10 @@ int topo_run(cpet_ctx* ctx, int N, int L, const float* starts,
   int G = ctx->tune[2][1];
   if (S <= 0) S = 16;  // 16 waves x 64 slots: fastest measured (S 4/8/16, G 256..2048)
   const int gmax = (L + 63) / 64;
-  if (G <= 0) G = std::min(gmax, std::max(1, ctx->num_cus * 2));
+  // One 16-wave workgroup per CU: the fewest slots that keep every SIMD fed
+  // (4 waves each, with the charge loads prefetched), so the tail of long
+  // streamlines still running after the queue drains is shortest.
+  if (G <= 0) G = std::min(gmax, std::max(1, ctx->num_cus * (S >= 16 ? 1 : 2)));
   G = std::min(G, gmax);
   hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(256), 0, st, d_counter, 1LL,
                      (unsigned int)(G * 64));
