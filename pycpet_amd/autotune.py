@@ -117,7 +117,7 @@ def tune_grid(ctx: nat.Context, axes, x, q, *, want_phi=False, cache: Optional[d
 
 
 def tune_batch_topo(ctx: nat.Context, frames, *, cache: Optional[dict] = None,
-                    waves=(4, 8, 16), groups=(0, 512, 1024, 2048)) -> Tuple[int, int]:
+                    waves=(4, 8, 16), groups=(0, 256, 512, 1024)) -> Tuple[int, int]:
     """(waves per workgroup, persistent workgroups) of the batched
     streamline launch, timed on `frames` (a representative subset)."""
     from . import batch as B

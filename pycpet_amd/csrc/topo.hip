@@ -146,52 +146,40 @@ struct PropArgs {
   unsigned int* counter;
 };
 
-// Unsoftened field of charge tiles [t0, t1) at one point, FP32 per tile and
-// FP64 across tiles (the k factor is applied by the caller).
-__device__ __forceinline__ void slice_field(float px, float py, float pz,
-                                            const float4* __restrict__ ch,
-                                            int t0, int t1, double& ex,
-                                            double& ey, double& ez) {
-  // Charges stream through two SGPR buffers of 4 (one 64-byte scalar load
-  // each).  The load of the next group is issued only after the current
-  // group has arrived (an explicit lgkmcnt(0) wait: scalar loads return out
-  // of order, so any later wait covers every load in flight), and a
-  // scheduling barrier keeps it ahead of the current group's 52 VALU
-  // instructions, which then hide its latency.  The stream runs on across
-  // tile boundaries; the last prefetch is clamped to the wave's last group,
-  // so no load leaves the wave's charge range.
+// Streams the records of source tiles [t0, t1) (kRec float4 per tile) to
+// body(c, fx, fy, fz), four float4 at a time, summing FP32 per tile and FP64
+// across tiles.  The records pass through two SGPR buffers of four float4
+// (one 64-byte scalar load each).  The load of the next group is issued
+// only after the current group has arrived (an explicit lgkmcnt(0) wait:
+// scalar loads return out of order, so any later wait covers every load in
+// flight), and scheduling barriers keep it ahead of the current group's
+// arithmetic, which then hides its latency.  The stream runs on across tile
+// boundaries; the last prefetch is clamped to the wave's last group, so no
+// load leaves the wave's range.
+template <int kRec, class Body>
+__device__ __forceinline__ void stream_tiles(const float4* __restrict__ ch, int t0, int t1,
+                                             Body body, double& ex, double& ey, double& ez) {
   constexpr int kGroup = 4;
   constexpr int kWaitLgkm0 = 0xc07f;  // s_waitcnt lgkmcnt(0), other counters untouched
+  static_assert(kRec % (2 * kGroup) == 0, "a tile holds whole group pairs");
   ex = ey = ez = 0.0;
   if (t0 >= t1) return;
-  const int last = t1 * kTile - kGroup;  // index of the wave's last group
-  int nx = t0 * kTile;
+  const int last = t1 * kRec - kGroup;  // index of the wave's last group
+  int nx = t0 * kRec;
   float4 a[kGroup], b[kGroup];
 #pragma unroll
   for (int u = 0; u < kGroup; ++u) a[u] = ch[nx + u];
-  auto pairs = [&](const float4 (&c)[kGroup], float& fx, float& fy, float& fz) {
-#pragma unroll
-    for (int u = 0; u < kGroup; ++u) {
-      const float dx = px - c[u].x, dy = py - c[u].y, dz = pz - c[u].z;
-      const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-      const float ri = __builtin_amdgcn_rsqf(r2);
-      const float s = (c[u].w * ri) * (ri * ri);
-      fx = fmaf(s, dx, fx);
-      fy = fmaf(s, dy, fy);
-      fz = fmaf(s, dz, fz);
-    }
-  };
   for (int t = t0; t < t1; ++t) {
     float fx = 0.f, fy = 0.f, fz = 0.f;
 #pragma unroll
-    for (int g = 0; g < kTile; g += 2 * kGroup) {
+    for (int g = 0; g < kRec; g += 2 * kGroup) {
       __builtin_amdgcn_sched_barrier(0);
       __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
       nx += kGroup;
 #pragma unroll
       for (int u = 0; u < kGroup; ++u) b[u] = ch[nx + u];
       __builtin_amdgcn_sched_barrier(0);
-      pairs(a, fx, fy, fz);
+      body(a, fx, fy, fz);
       __builtin_amdgcn_sched_barrier(0);
       __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
       nx += kGroup;
@@ -199,7 +187,7 @@ __device__ __forceinline__ void slice_field(float px, float py, float pz,
 #pragma unroll
       for (int u = 0; u < kGroup; ++u) a[u] = ch[pf + u];
       __builtin_amdgcn_sched_barrier(0);
-      pairs(b, fx, fy, fz);
+      body(b, fx, fy, fz);
     }
     ex += (double)fx;
     ey += (double)fy;
@@ -212,9 +200,22 @@ __device__ __forceinline__ void slice_field(float px, float py, float pz,
 //
 // Charges: one float4 (x, y, z, q) per source (math_module.c:296-333).
 struct ChargeSrc {
+  // Unsoftened field of charge tiles [t0, t1) at one point (the k factor is
+  // applied by the caller).
   __device__ static void slice(float px, float py, float pz, const float4* __restrict__ ch,
                                int t0, int t1, double& ex, double& ey, double& ez) {
-    slice_field(px, py, pz, ch, t0, t1, ex, ey, ez);
+    stream_tiles<kTile>(ch, t0, t1, [&](const float4 (&c)[4], float& fx, float& fy, float& fz) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float dx = px - c[u].x, dy = py - c[u].y, dz = pz - c[u].z;
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        const float ri = __builtin_amdgcn_rsqf(r2);
+        const float s = (c[u].w * ri) * (ri * ri);
+        fx = fmaf(s, dx, fx);
+        fy = fmaf(s, dy, fy);
+        fz = fmaf(s, dz, fz);
+      }
+    }, ex, ey, ez);
   }
   __device__ static void field2_f64(const double (&p)[2][3], double (&e)[2][3],
                                     const float4* __restrict__ ch, int npad) {
@@ -254,33 +255,21 @@ struct DipoleSrc {
   }
   __device__ static void slice(float px, float py, float pz, const float4* __restrict__ ch,
                                int t0, int t1, double& ex, double& ey, double& ez) {
-    constexpr int kGroup = 4;  // 4 dipoles = 8 float4 = two 64-byte scalar loads
-    ex = ey = ez = 0.0;
-    for (int t = t0; t < t1; ++t) {
-      float fx = 0.f, fy = 0.f, fz = 0.f;
-      const float4* cp = ch + 2 * t * kTile;
-      for (int g = 0; g < kTile; g += kGroup) {
-        float4 c[2 * kGroup];
+    stream_tiles<2 * kTile>(ch, t0, t1, [&](const float4 (&c)[4], float& fx, float& fy, float& fz) {
 #pragma unroll
-        for (int u = 0; u < 2 * kGroup; ++u) c[u] = cp[2 * g + u];
-#pragma unroll
-        for (int u = 0; u < kGroup; ++u) {
-          const float4 X = c[2 * u], M = c[2 * u + 1];
-          const float dx = px - X.x, dy = py - X.y, dz = pz - X.z;
-          const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-          const float ri = __builtin_amdgcn_rsqf(r2);
-          const float ri2 = ri * ri;
-          const float ri5 = (ri2 * ri2) * ri;
-          const float cc = coef(M.x, M.y, M.z, dx, dy, dz);
-          fx = fmaf(ri5, fmaf(cc, dx, -M.x * r2), fx);
-          fy = fmaf(ri5, fmaf(cc, dy, -M.y * r2), fy);
-          fz = fmaf(ri5, fmaf(cc, dz, -M.z * r2), fz);
-        }
+      for (int u = 0; u < 2; ++u) {
+        const float4 X = c[2 * u], M = c[2 * u + 1];
+        const float dx = px - X.x, dy = py - X.y, dz = pz - X.z;
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        const float ri = __builtin_amdgcn_rsqf(r2);
+        const float ri2 = ri * ri;
+        const float ri5 = (ri2 * ri2) * ri;
+        const float cc = coef(M.x, M.y, M.z, dx, dy, dz);
+        fx = fmaf(ri5, fmaf(cc, dx, -M.x * r2), fx);
+        fy = fmaf(ri5, fmaf(cc, dy, -M.y * r2), fy);
+        fz = fmaf(ri5, fmaf(cc, dz, -M.z * r2), fz);
       }
-      ex += (double)fx;
-      ey += (double)fy;
-      ez += (double)fz;
-    }
+    }, ex, ey, ez);
   }
   __device__ static void field2_f64(const double (&p)[2][3], double (&e)[2][3],
                                     const float4* __restrict__ ch, int npad) {
@@ -913,7 +902,7 @@ int topo_batch_run(cpet_ctx* ctx, int F, const int32_t* n_starts, const float* s
   }
 
   int S = ctx->tune[3][0] > 0 ? ctx->tune[3][0] : 16;  // as the single-frame kernel: same results
-  int G = ctx->tune[3][1] > 0 ? ctx->tune[3][1] : ctx->num_cus * 2;
+  int G = ctx->tune[3][1] > 0 ? ctx->tune[3][1] : ctx->num_cus * (S >= 16 ? 1 : 2);
   G = std::max(1, std::min<int>(G, (int)assign.size()));
   BatchTopoArgs ba;
   ba.F = F;
