@@ -131,6 +131,75 @@ __global__ __launch_bounds__(1024) void bucket_scatter_kernel(
   if (l < L) order[base[b] + rank] = l;
 }
 
+// Per-frame longest-first order for batches: one counting sort over
+// (frame, n_iter class) keys in global memory, frame-major, so each frame's
+// streamlines land in its own range [sl_off[f], sl_off[f+1]) as local ids.
+// 512 classes of 8 steps per frame (n_iter >= 4095 share the first).
+constexpr int kBatchBuckets = 512;
+
+__device__ __forceinline__ int batch_bucket_of(int n) {
+  n = n < 0 ? 0 : (n > 4095 ? 4095 : n);
+  return kBatchBuckets - 1 - (n >> 3);  // descending n_iter
+}
+
+// The frame holding streamline l: the largest f with off[f] <= l.  Every
+// frame has at least one streamline, so off is strictly increasing.
+__device__ __forceinline__ int frame_of(const int32_t* __restrict__ off, int F, int l) {
+  int lo = 0, hi = F - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= l)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void batch_bucket_count_kernel(
+    int Lt, int F, const int32_t* __restrict__ sl_off, const int32_t* __restrict__ n_iter,
+    unsigned int* counts) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= Lt) return;
+  const int f = frame_of(sl_off, F, l);
+  atomicAdd(&counts[(size_t)f * kBatchBuckets + batch_bucket_of(n_iter[l])], 1u);
+}
+
+// Exclusive scan of n counts in place by one 1024-thread block.
+__global__ __launch_bounds__(1024) void scan_u32_kernel(unsigned int* a, int n) {
+  __shared__ unsigned int sums[1024];
+  const int t = threadIdx.x;
+  const int per = (n + 1023) / 1024;
+  const int b0 = min(t * per, n), b1 = min(b0 + per, n);
+  unsigned int local = 0;
+  for (int i = b0; i < b1; ++i) local += a[i];
+  sums[t] = local;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const unsigned int v = t >= off ? sums[t - off] : 0u;
+    __syncthreads();
+    sums[t] += v;
+    __syncthreads();
+  }
+  unsigned int run = sums[t] - local;
+  for (int i = b0; i < b1; ++i) {
+    const unsigned int c = a[i];
+    a[i] = run;
+    run += c;
+  }
+}
+
+__global__ __launch_bounds__(256) void batch_bucket_scatter_kernel(
+    int Lt, int F, const int32_t* __restrict__ sl_off, const int32_t* __restrict__ n_iter,
+    unsigned int* offsets, int32_t* __restrict__ order) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= Lt) return;
+  const int f = frame_of(sl_off, F, l);
+  const unsigned int pos =
+      atomicAdd(&offsets[(size_t)f * kBatchBuckets + batch_bucket_of(n_iter[l])], 1u);
+  order[pos] = l - sl_off[f];
+}
+
 // ---- 2. persistent propagation ----------------------------------------
 
 struct PropArgs {
@@ -449,6 +518,7 @@ struct BatchTopoArgs {
   int32_t* steps;             // (sum L)
   int32_t* exited;            // (sum L)
   unsigned int* sl_counter;   // (F) per-frame queue positions, start at 0
+  const int32_t* order;       // (sum L) per-frame longest-first local ids
   const int32_t* assign;      // frame id per assignment
   int n_assign;
   unsigned int* assign_counter;
@@ -477,7 +547,7 @@ __global__ __launch_bounds__(64 * S, 8) void topo_batch_kernel(const float4* __r
     f.L = a.sl_off[fr + 1] - s0;
     f.starts = a.starts + 3LL * s0;
     f.n_iter = a.n_iter + s0;
-    f.order = nullptr;
+    f.order = a.order + s0;
     f.h = a.hdims[4 * fr];
     f.d0 = a.hdims[4 * fr + 1];
     f.d1 = a.hdims[4 * fr + 2];
@@ -835,7 +905,15 @@ int topo_batch_run(cpet_ctx* ctx, int F, const int32_t* n_starts, const float* s
   int32_t* d_steps = (dev && steps) ? steps : (int32_t*)(d_fin + 5 * (size_t)Lt);
   int32_t* d_exited = (dev && exited) ? exited : (int32_t*)(d_fin + 5 * (size_t)Lt) + Lt;
 
-  // assignments: frames by decreasing work, one entry per 64 streamlines
+  const int S = ctx->tune[3][0] > 0 ? ctx->tune[3][0] : 16;  // as the single-frame kernel: same results
+  const int G0 = ctx->tune[3][1] > 0 ? ctx->tune[3][1] : ctx->num_cus * (S >= 16 ? 1 : 2);
+  // Assignments: frames by decreasing work, one entry per A streamlines.
+  // A workgroup that takes an assignment runs its 64 slots on that frame
+  // until the frame's queue is empty, then drains them; larger assignments
+  // put fewer workgroups on a frame, each running more streamlines per slot,
+  // so the drain is a smaller share of its work.  A grows from 64 to 256
+  // while the batch still gives every workgroup four or more assignments.
+  const int A = 64 * std::max(1, std::min(4, (int)(Lt / ((long long)G0 * 64 * 4))));
   std::vector<int> byc(F);
   for (int f = 0; f < F; ++f) byc[f] = f;
   std::sort(byc.begin(), byc.end(), [&](int a, int b) {
@@ -844,7 +922,7 @@ int topo_batch_run(cpet_ctx* ctx, int F, const int32_t* n_starts, const float* s
   });
   std::vector<int32_t> assign;
   for (int f : byc)
-    for (int r = 0; r < (n_starts[f] + 63) / 64; ++r) assign.push_back(f);
+    for (int r = 0; r < (n_starts[f] + A - 1) / A; ++r) assign.push_back(f);
   std::vector<int32_t> blk;
   for (int f = 0; f < F; ++f)
     for (int b = 0; b < (n_starts[f] + 255) / 256; ++b) {
@@ -876,7 +954,10 @@ int topo_batch_run(cpet_ctx* ctx, int F, const int32_t* n_starts, const float* s
   const size_t o_cnt = 0;
   const size_t o_hist = ((F + 1) * sizeof(uint32_t) + 255) & ~(size_t)255;
   const size_t o_edges = (o_hist + (size_t)F * nbins * sizeof(uint32_t) + 255) & ~(size_t)255;
-  const size_t total = o_edges + (hist ? (size_t)(nb_d + nb_c + 2) * sizeof(double) : 0) + 256;
+  const size_t o_bk = (o_edges + (hist ? (size_t)(nb_d + nb_c + 2) * sizeof(double) : 0) + 255) &
+                      ~(size_t)255;
+  const size_t o_ord = (o_bk + (size_t)F * kBatchBuckets * sizeof(uint32_t) + 255) & ~(size_t)255;
+  const size_t total = o_ord + (size_t)Lt * sizeof(int32_t) + 256;
   void* work = nullptr;
   if ((rc = ctx_buf(ctx, kSlotBatchWork, total, &work))) return rc;
   char* w = (char*)work;
@@ -901,9 +982,20 @@ int topo_batch_run(cpet_ctx* ctx, int F, const int32_t* n_starts, const float* s
     }
   }
 
-  int S = ctx->tune[3][0] > 0 ? ctx->tune[3][0] : 16;  // as the single-frame kernel: same results
-  int G = ctx->tune[3][1] > 0 ? ctx->tune[3][1] : ctx->num_cus * (S >= 16 ? 1 : 2);
-  G = std::max(1, std::min<int>(G, (int)assign.size()));
+  // per-frame longest-first order
+  unsigned int* d_bk = (unsigned int*)(w + o_bk);
+  int32_t* d_ord = (int32_t*)(w + o_ord);
+  const long long nbk = (long long)F * kBatchBuckets;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((nbk + 255) / 256)), dim3(256), 0, st, d_bk,
+                     nbk, 0u);
+  hipLaunchKernelGGL(batch_bucket_count_kernel, dim3((Lt + 255) / 256), dim3(256), 0, st, Lt, F,
+                     (const int32_t*)(tw + o_sl), (const int32_t*)diter, d_bk);
+  hipLaunchKernelGGL(scan_u32_kernel, dim3(1), dim3(1024), 0, st, d_bk, (int)nbk);
+  hipLaunchKernelGGL(batch_bucket_scatter_kernel, dim3((Lt + 255) / 256), dim3(256), 0, st, Lt,
+                     F, (const int32_t*)(tw + o_sl), (const int32_t*)diter, d_bk, d_ord);
+  CPET_HIP_TRY(hipGetLastError());
+
+  const int G = std::max(1, std::min<int>(G0, (int)assign.size()));
   BatchTopoArgs ba;
   ba.F = F;
   ba.chg_off = d_chg_off;
@@ -915,6 +1007,7 @@ int topo_batch_run(cpet_ctx* ctx, int F, const int32_t* n_starts, const float* s
   ba.steps = d_steps;
   ba.exited = d_exited;
   ba.sl_counter = d_cnt;
+  ba.order = d_ord;
   ba.assign = (const int32_t*)(tw + o_as);
   ba.n_assign = (int)assign.size();
   ba.assign_counter = d_cnt + F;
