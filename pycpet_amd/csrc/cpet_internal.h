@@ -126,6 +126,7 @@ enum Slot : int {
   kSlotCluster0,     // cluster step: staged features / histograms
   kSlotCluster1,     // cluster step: edges, offsets, counts
   kSlotCluster2,     // cluster step: output
+  kSlotNear,         // grid kernels: charge-near-grid flags
   kSlotCount
 };
 

@@ -253,7 +253,42 @@ struct FieldOut {
   long long M;
   int* fix_count;  // auto-mode check: number of listed points, or null
   long long* fix_list;  // output indices of points to recompute
+  const int* near;  // softened E: 0 when no charge is near a grid point, or null
 };
+
+// The r^2 >= 1e-6 floor of the volume field can only bind for a charge
+// within 1e-3 A of a grid point.  kNearGrid adds 1 % for the FP32 rounding
+// of p - x (|p| up to ~1e3 A), so below it the floored and unfloored sums
+// are the same numbers.
+constexpr double kNearGrid = 1.01e-3;
+
+__device__ __forceinline__ double nearest_sq(const float* __restrict__ a, int n, float v) {
+  double best = 1e300;
+  for (int i = 0; i < n; ++i) {
+    const double d = (double)a[i] - (double)v;
+    best = fmin(best, d * d);
+  }
+  return best;
+}
+
+// *flag = 1 when some charge lies within kNearGrid of a grid point; the
+// caller clears it first (near_clear_kernel).  One charge per thread with
+// linear scans over the axes, so they need not be sorted; the nearest grid
+// point of a rectilinear grid is the product of the nearest axis values.
+// Every writer stores the same 1, so the stores need no atomics.
+__global__ __launch_bounds__(256) void near_grid_kernel(int n, const float4* __restrict__ ch,
+                                                        int nx, const float* __restrict__ ax,
+                                                        int ny, const float* __restrict__ ay,
+                                                        int nz, const float* __restrict__ az,
+                                                        int* flag) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const float4 c = ch[j];
+  const double d2 = nearest_sq(ax, nx, c.x) + nearest_sq(ay, ny, c.y) + nearest_sq(az, nz, c.z);
+  if (d2 < kNearGrid * kNearGrid) *flag = 1;
+}
+
+__global__ void near_clear_kernel(int* flag) { *flag = 0; }
 
 template <int WHAT>
 __device__ __forceinline__ void store_point(const FieldOut& o, long long m,
@@ -331,7 +366,14 @@ __global__ __launch_bounds__(256) void field_grid_kernel(
   const int t0 = split * tiles_per_split;
   const int t1 = min(t0 + tiles_per_split, ntiles);
   double acc[K][kAcc];
-  accumulate<K, true, MODE, WHAT>(px, py, pz, ch, chd, t0, t1, acc);
+  if constexpr (WHAT == kWhatESoft) {
+    if (o.near && *o.near == 0)  // no charge near a grid point: the floor never binds
+      accumulate<K, true, MODE, kWhatEHard>(px, py, pz, ch, chd, t0, t1, acc);
+    else
+      accumulate<K, true, MODE, WHAT>(px, py, pz, ch, chd, t0, t1, acc);
+  } else {
+    accumulate<K, true, MODE, WHAT>(px, py, pz, ch, chd, t0, t1, acc);
+  }
   const long long base = line * nz + z0;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -544,6 +586,7 @@ __global__ __launch_bounds__(256) void field_batch_kernel(const float4* __restri
   o.M = (long long)nx * ny * nz;
   o.fix_count = a.fix_count;
   o.fix_list = a.fix_list;
+  o.near = nullptr;
   const long long base = line * nz + z0;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -711,6 +754,7 @@ int field_launch(cpet_ctx* ctx, bool grid_mode, int nx, int ny, int nz,
   o.M = M;
   o.fix_count = nullptr;
   o.fix_list = nullptr;
+  o.near = nullptr;
   ctx->fix_count = nullptr;
 
   int K;
@@ -750,6 +794,16 @@ int field_launch(cpet_ctx* ctx, bool grid_mode, int nx, int ny, int nz,
   if (check) {
     const int rc = fix_prepare(ctx, M, &o.fix_count, &o.fix_list);
     if (rc) return rc;
+  }
+  if (grid_mode && what == kWhatESoft) {
+    void* nf = nullptr;
+    const int rc = ctx_buf(ctx, kSlotNear, sizeof(int), &nf);
+    if (rc) return rc;
+    hipLaunchKernelGGL(near_clear_kernel, dim3(1), dim3(1), 0, ctx->stream, (int*)nf);
+    hipLaunchKernelGGL(near_grid_kernel, dim3((npad + 255) / 256), dim3(256), 0, ctx->stream,
+                       npad, ch, nx, ax, ny, ay, nz, az, (int*)nf);
+    CPET_HIP_TRY(hipGetLastError());
+    o.near = (const int*)nf;
   }
   const long long blocks_x = grid_mode ? (lanes + 255) / 256
                                        : (M + 256LL * K - 1) / (256LL * K);

@@ -142,6 +142,31 @@ def test_edge_cases(gpu_ctx, oracle_mod):
         ops.field_grid(np.zeros(0, np.float32), axes[1], axes[2], x, q, ctx=gpu_ctx)
 
 
+def test_floor_near_grid_points(gpu_ctx, oracle_mod):
+    """The volume field's r^2 >= 1e-6 floor (math_module.c:405-451) with E
+    alone: the grid kernel skips the floor only when no charge is within
+    1.01e-3 A of a grid point, so results must match the floored oracle both
+    when a charge sits on or next to a grid point and when none does."""
+    from pycpet_amd import ops
+
+    rng = np.random.default_rng(5)
+    axes = [np.linspace(-1, 1, 21, dtype=np.float32)] * 3
+    far = rng.uniform(-3, 3, (200, 3)).astype(np.float32)
+    q = rng.normal(0, 0.5, 201).astype(np.float32)
+    g = np.array([0.1, 0.2, -0.3], np.float32)  # a grid point (step 0.1)
+    for offset in (0.0, 5e-4, 9.9e-4, 1.2e-3, 2e-3):
+        x = np.concatenate([far, (g + np.float32(offset) * np.array([1, 0, 0], np.float32))[None]])
+        E, _ = ops.field_grid(*axes, x, q, ctx=gpu_ctx)
+        E64, _ = oracle_mod.field_f64(mesh(axes), x, q)
+        assert np.all(np.isfinite(E)), offset
+        assert rel_err(E, E64)[0] <= TOL, offset
+    # no charge near the grid: the unfloored sum equals the floored oracle
+    x = np.concatenate([far, np.array([[0.15, 0.25, -0.35]], np.float32)])
+    E, _ = ops.field_grid(*axes, x, q, ctx=gpu_ctx)
+    E64, _ = oracle_mod.field_f64(mesh(axes), x, q)
+    assert rel_err(E, E64)[0] <= TOL
+
+
 def test_charge_split_invariance(gpu_ctx):
     """Splitting the charge sum over workgroups only regroups FP64 partials."""
     from pycpet_amd import ops, workloads as W
