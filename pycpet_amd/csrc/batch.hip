@@ -11,7 +11,7 @@
 namespace cpet {
 int field_batch_launch(cpet_ctx* ctx, int F, const int32_t* geo3, const float* axes_dev,
                        const float4* ch, const double4* chd, const int32_t* d_chg_off,
-                       void* E_out, void* phi_out, int soften, uint32_t flags);
+                       int ch_total, void* E_out, void* phi_out, int soften, uint32_t flags);
 int topo_batch_run(cpet_ctx* ctx, int F, const int32_t* n_starts, const float* starts,
                    const int32_t* n_iter, const float* h, const float* dims, const float4* ch,
                    const int32_t* d_chg_off, const std::vector<int32_t>& chg_off, double* feats,
@@ -59,6 +59,7 @@ int cpet_batch_volume(cpet_ctx* ctx, int F, const int32_t* n_charges, const floa
     if (phi_out && (rc = ctx_buf(ctx, kSlotOut1, (size_t)M * osz, &dphi))) return rc;
   }
   rc = field_batch_launch(ctx, F, grid_dims, (const float*)daxes, ch, chd, d_chg_off,
+                          chg_off.empty() ? 0 : chg_off.back(),
                           E_out ? dE : nullptr, phi_out ? dphi : nullptr, soften, flags);
   if (rc) return rc;
   if (!dev) {

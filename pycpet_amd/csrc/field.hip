@@ -290,6 +290,38 @@ __global__ __launch_bounds__(256) void near_grid_kernel(int n, const float4* __r
 
 __global__ void near_clear_kernel(int* flag) { *flag = 0; }
 
+__global__ __launch_bounds__(256) void near_clear_n_kernel(int* flags, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flags[i] = 0;
+}
+
+// Batches: flags[f] = 1 when a charge of frame f lies within kNearGrid of a
+// point of frame f's grid.  chg_off (F+1) holds the padded charge offsets,
+// geo (F,4) nx, ny, nz, nzc and ax_off (F) the start of each frame's axes.
+__global__ __launch_bounds__(256) void near_batch_kernel(int n, const float4* __restrict__ ch,
+                                                         int F, const int32_t* __restrict__ chg_off,
+                                                         const int32_t* __restrict__ geo,
+                                                         const int32_t* __restrict__ ax_off,
+                                                         const float* __restrict__ axes,
+                                                         int* flags) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  int lo = 0, hi = F - 1;  // the last frame f with chg_off[f] <= j
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (chg_off[mid] <= j)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const int nx = geo[4 * lo], ny = geo[4 * lo + 1], nz = geo[4 * lo + 2];
+  const float* ax = axes + ax_off[lo];
+  const float4 c = ch[j];
+  const double d2 = nearest_sq(ax, nx, c.x) + nearest_sq(ax + nx, ny, c.y) +
+                    nearest_sq(ax + nx + ny, nz, c.z);
+  if (d2 < kNearGrid * kNearGrid) flags[lo] = 1;
+}
+
 template <int WHAT>
 __device__ __forceinline__ void store_point(const FieldOut& o, long long m,
                                             const double (&acc)[kAcc], int split) {
@@ -547,6 +579,7 @@ struct BatchFieldArgs {
   int f64;
   int* fix_count;
   long long* fix_list;
+  const int* near;  // (F) softened E: 0 when no charge is near the frame's grid, or null
 };
 
 template <int K, int MODE, int WHAT>
@@ -575,7 +608,15 @@ __global__ __launch_bounds__(256) void field_batch_kernel(const float4* __restri
   const int c0 = a.chg_off[fr];
   const int ntiles = (a.chg_off[fr + 1] - c0) / kTile;
   double acc[K][kAcc];
-  accumulate<K, true, MODE, WHAT>(px, py, pz, ch + c0, chd ? chd + c0 : nullptr, 0, ntiles, acc);
+  const double4* cd = chd ? chd + c0 : nullptr;
+  if constexpr (WHAT == kWhatESoft) {
+    if (a.near && a.near[fr] == 0)  // no charge near this grid: the floor never binds
+      accumulate<K, true, MODE, kWhatEHard>(px, py, pz, ch + c0, cd, 0, ntiles, acc);
+    else
+      accumulate<K, true, MODE, WHAT>(px, py, pz, ch + c0, cd, 0, ntiles, acc);
+  } else {
+    accumulate<K, true, MODE, WHAT>(px, py, pz, ch + c0, cd, 0, ntiles, acc);
+  }
   FieldOut o;
   const size_t es = a.f64 ? sizeof(double) : sizeof(float);
   const long long off = a.out_off[fr];
@@ -878,7 +919,7 @@ int field_launch(cpet_ctx* ctx, bool grid_mode, int nx, int ny, int nz,
 // nz per frame (geo3, F*3), charges already packed (chg_off).
 int field_batch_launch(cpet_ctx* ctx, int F, const int32_t* geo3, const float* axes_dev,
                        const float4* ch, const double4* chd, const int32_t* d_chg_off,
-                       void* E_out, void* phi_out, int soften, uint32_t flags) {
+                       int ch_total, void* E_out, void* phi_out, int soften, uint32_t flags) {
   const int what = what_of(E_out != nullptr, phi_out != nullptr, soften);
   if (what < 0) {
     set_error("E and phi together require soften=1 (volume semantics)");
@@ -940,7 +981,17 @@ int field_batch_launch(cpet_ctx* ctx, int F, const int32_t* geo3, const float* a
   a.f64 = (flags & CPET_F_OUT_F64) ? 1 : 0;
   a.fix_count = nullptr;
   a.fix_list = nullptr;
+  a.near = nullptr;
   ctx->fix_count = nullptr;
+  if (what == kWhatESoft && ch_total > 0) {
+    void* nf = nullptr;
+    if ((rc = ctx_buf(ctx, kSlotNear, (size_t)F * sizeof(int), &nf))) return rc;
+    hipLaunchKernelGGL(near_clear_n_kernel, dim3((F + 255) / 256), dim3(256), 0, st, (int*)nf, F);
+    hipLaunchKernelGGL(near_batch_kernel, dim3((ch_total + 255) / 256), dim3(256), 0, st,
+                       ch_total, ch, F, d_chg_off, a.geo, a.ax_off, axes_dev, (int*)nf);
+    CPET_HIP_TRY(hipGetLastError());
+    a.near = (const int*)nf;
+  }
   const bool check = mode == kModeAuto && E_out != nullptr;
   if (check && (rc = fix_prepare(ctx, pts, &a.fix_count, &a.fix_list))) return rc;
   const int nblk = (int)(blk.size() / 2);

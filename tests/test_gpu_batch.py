@@ -55,6 +55,34 @@ def test_batch_volume_equals_single_frames(gpu_ctx, oracle_mod):
     assert err.max() <= 1e-5
 
 
+def test_batch_volume_floor_near_grid(gpu_ctx, oracle_mod):
+    """E alone (the floored volume field): frames with a charge on or next to
+    a grid point keep the r^2 >= 1e-6 floor, the others skip it; each frame
+    equals the frame run alone, and the near frames match the oracle."""
+    from pycpet_amd import batch, ops
+
+    frames = [f for f in _small_frames(12, seed=3) if f["kind"] == "volume"]
+    for i, f in enumerate(frames[:3]):  # frames 0-2: a charge near a grid point
+        g = np.array([f["axes"][0][1], f["axes"][1][2], f["axes"][2][0]], np.float32)
+        near = g + np.float32((0.0, 4e-4, 9e-4)[i]) * np.array([0, 0, 1], np.float32)
+        f["x"] = np.concatenate([f["x"], near[None]]).astype(np.float32)
+        f["q"] = np.concatenate([f["q"], np.float32([0.4])]).astype(np.float32)
+    gpu_ctx.set_tuning(4, 2, 0)
+    gpu_ctx.set_tuning(0, 2, 1)  # single-frame: same K, no charge split
+    outs = batch.batch_volume(frames, out_f64=True, ctx=gpu_ctx)
+    for f, (E, _) in zip(frames, outs):
+        E1, _ = ops.field_grid(*f["axes"], f["x"], f["q"], out_f64=True, ctx=gpu_ctx)
+        np.testing.assert_array_equal(E, E1)
+    gpu_ctx.set_tuning(4, 0, 0)
+    gpu_ctx.set_tuning(0, 0, 0)
+    for f, (E, _) in list(zip(frames, outs))[:3]:
+        pts = np.stack(np.meshgrid(*f["axes"], indexing="ij"), -1).reshape(-1, 3)
+        E64, _ = oracle_mod.field_f64(pts, f["x"], f["q"])
+        nE = np.linalg.norm(E64, axis=1)
+        err = np.linalg.norm(E - E64, axis=1) / np.maximum(nE, 1e-3 * np.sqrt(np.mean(nE ** 2)))
+        assert np.all(np.isfinite(E)) and err.max() <= 1e-5
+
+
 def test_batch_topo_equals_single_frames(gpu_ctx, oracle_mod):
     from pycpet_amd import batch, ops
 
