@@ -492,74 +492,118 @@ struct FixArgs {
   int f64;
 };
 
+// Grid point, charge range [c0, c1) of the listed output index g.
+__device__ __forceinline__ void fix_point(const FixArgs& a, long long g, double& px, double& py,
+                                          double& pz, int& c0, int& c1) {
+  c0 = 0;
+  c1 = a.npad;
+  if (a.F > 0) {
+    int lo = 0, hi = a.F - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (a.out_off[mid] <= g) lo = mid; else hi = mid - 1;
+    }
+    const long long m = g - a.out_off[lo];
+    const int nx = a.geo[4 * lo], ny = a.geo[4 * lo + 1], nz = a.geo[4 * lo + 2];
+    const long long line = m / nz;
+    const int z = (int)(m - line * nz), i = (int)(line / ny), j = (int)(line - (long long)i * ny);
+    const float* ax = a.axes + a.ax_off[lo];
+    px = ax[i];
+    py = ax[nx + j];
+    pz = ax[nx + ny + z];
+    c0 = a.chg_off[lo];
+    c1 = a.chg_off[lo + 1];
+  } else if (a.pts) {
+    px = a.pts[3 * g];
+    py = a.pts[3 * g + 1];
+    pz = a.pts[3 * g + 2];
+  } else {
+    const long long line = g / a.nz;
+    const int z = (int)(g - line * a.nz), i = (int)(line / a.ny),
+              j = (int)(line - (long long)i * a.ny);
+    px = a.ax[i];
+    py = a.ay[j];
+    pz = a.az[z];
+  }
+}
+
+// One wave refines kFixPts listed points at once: lanes split the charges,
+// every charge read from L2 serves all kFixPts points (the kernel is bound
+// by that traffic: each point reads every charge), and the wave's partial
+// sums are reduced across lanes in a fixed order.  Points of one group that
+// lie in different frames of a batch (different charge ranges) are refined
+// one at a time.
+constexpr int kFixPts = 4;
+
 template <bool SOFT>
 __global__ __launch_bounds__(256) void field_fix_kernel(FixArgs a) {
   const int n = *a.count;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const double rclampd = rsqrt_f64((double)1e-6f);
-  for (long long t = (long long)blockIdx.x * 4 + wave; t < n; t += (long long)gridDim.x * 4) {
-    const long long g = a.list[t];
-    float px, py, pz;
-    int c0 = 0, c1 = a.npad;
-    if (a.F > 0) {
-      int lo = 0, hi = a.F - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (a.out_off[mid] <= g) lo = mid; else hi = mid - 1;
-      }
-      const long long m = g - a.out_off[lo];
-      const int nx = a.geo[4 * lo], ny = a.geo[4 * lo + 1], nz = a.geo[4 * lo + 2];
-      const long long line = m / nz;
-      const int z = (int)(m - line * nz), i = (int)(line / ny), j = (int)(line - (long long)i * ny);
-      const float* ax = a.axes + a.ax_off[lo];
-      px = ax[i];
-      py = ax[nx + j];
-      pz = ax[nx + ny + z];
-      c0 = a.chg_off[lo];
-      c1 = a.chg_off[lo + 1];
-    } else if (a.pts) {
-      px = a.pts[3 * g];
-      py = a.pts[3 * g + 1];
-      pz = a.pts[3 * g + 2];
-    } else {
-      const long long line = g / a.nz;
-      const int z = (int)(g - line * a.nz), i = (int)(line / a.ny),
-                j = (int)(line - (long long)i * a.ny);
-      px = a.ax[i];
-      py = a.ay[j];
-      pz = a.az[z];
+  auto pair = [&](const float4& c, double px, double py, double pz, double (&e)[3]) {
+    const double dx = px - (double)c.x, dy = py - (double)c.y, dz = pz - (double)c.z;
+    const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+    const double y = rsqrt_refine(__builtin_amdgcn_rsqf((float)r2), r2);
+    const double re = SOFT ? fmin(y, rclampd) : y;
+    const double s = ((double)c.w * re) * (re * re);
+    e[0] = fma(s, dx, e[0]);
+    e[1] = fma(s, dy, e[1]);
+    e[2] = fma(s, dz, e[2]);
+  };
+  for (long long t0 = ((long long)blockIdx.x * 4 + wave) * kFixPts; t0 < n;
+       t0 += (long long)gridDim.x * 4 * kFixPts) {
+    const int np = (int)min((long long)kFixPts, (long long)n - t0);
+    long long g[kFixPts];
+    double px[kFixPts], py[kFixPts], pz[kFixPts];
+    int c0[kFixPts], c1[kFixPts];
+    bool same = true;
+#pragma unroll
+    for (int p = 0; p < kFixPts; ++p) {
+      g[p] = a.list[t0 + min(p, np - 1)];  // short groups repeat the last point
+      fix_point(a, g[p], px[p], py[p], pz[p], c0[p], c1[p]);
+      same = same && c0[p] == c0[0] && c1[p] == c1[0];
     }
-    const double pxd = px, pyd = py, pzd = pz;
-    double e0 = 0.0, e1 = 0.0, e2 = 0.0;
-    for (int jq = c0 + lane; jq < c1; jq += 64) {
-      const float4 c = a.ch[jq];
-      const double dx = pxd - (double)c.x, dy = pyd - (double)c.y, dz = pzd - (double)c.z;
-      const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-      const double y = rsqrt_refine(__builtin_amdgcn_rsqf((float)r2), r2);
-      const double re = SOFT ? fmin(y, rclampd) : y;
-      const double s = ((double)c.w * re) * (re * re);
-      e0 = fma(s, dx, e0);
-      e1 = fma(s, dy, e1);
-      e2 = fma(s, dz, e2);
+    double e[kFixPts][3] = {};
+    if (same) {
+      for (int jq = c0[0] + lane; jq < c1[0]; jq += 64) {
+        const float4 c = a.ch[jq];
+#pragma unroll
+        for (int p = 0; p < kFixPts; ++p) pair(c, px[p], py[p], pz[p], e[p]);
+      }
+    } else {
+#pragma unroll
+      for (int p = 0; p < kFixPts; ++p)
+        for (int jq = c0[p] + lane; jq < c1[p]; jq += 64) pair(a.ch[jq], px[p], py[p], pz[p], e[p]);
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      e0 += __shfl_xor(e0, off);
-      e1 += __shfl_xor(e1, off);
-      e2 += __shfl_xor(e2, off);
-    }
-    if (lane == 0) {
+    for (int p = 0; p < kFixPts; ++p)
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        e[p][0] += __shfl_xor(e[p][0], off);
+        e[p][1] += __shfl_xor(e[p][1], off);
+        e[p][2] += __shfl_xor(e[p][2], off);
+      }
+    if (lane < np) {  // lane p stores point p
+      double v[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+      for (int p = 0; p < kFixPts; ++p)
+        if (p == lane) {
+          v[0] = e[p][0];
+          v[1] = e[p][1];
+          v[2] = e[p][2];
+        }
+      const long long gp = a.list[t0 + lane];
       const double k = kCoulomb;
       if (a.f64) {
-        double* e = (double*)a.e + 3 * g;
-        e[0] = k * e0;
-        e[1] = k * e1;
-        e[2] = k * e2;
+        double* o = (double*)a.e + 3 * gp;
+        o[0] = k * v[0];
+        o[1] = k * v[1];
+        o[2] = k * v[2];
       } else {
-        float* e = (float*)a.e + 3 * g;
-        e[0] = (float)(k * e0);
-        e[1] = (float)(k * e1);
-        e[2] = (float)(k * e2);
+        float* o = (float*)a.e + 3 * gp;
+        o[0] = (float)(k * v[0]);
+        o[1] = (float)(k * v[1]);
+        o[2] = (float)(k * v[2]);
       }
     }
   }
