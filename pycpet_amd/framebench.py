@@ -56,7 +56,11 @@ def run_batch(n_frames: int, topo_starts: int, seed: int = 3) -> dict:
     ctx = nat.Context(0)
 
     def per_frame(plain: bool):
-        for f in dframes:
+        t0 = time.perf_counter()
+        for i, f in enumerate(dframes):
+            if i and i % 64 == 0:  # progress for long runs (512 frames plain: minutes)
+                print(f"{'plain' if plain else 'per_frame'}: {i}/{len(dframes)} frames, "
+                      f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
             if f["kind"] == "volume":
                 if plain:
                     ctx.set_tuning(0, 1, 1)
@@ -116,9 +120,7 @@ def run_batch(n_frames: int, topo_starts: int, seed: int = 3) -> dict:
         torch.cuda.synchronize()
         timings[name] = time.perf_counter() - t0
         print(f"{name}: {timings[name]:.3f} s", file=sys.stderr, flush=True)
-    steps = sum(int(r.steps.sum().item()) for r in results["top"])
     vpairs = sum(int(np.prod([len(a) for a in f["axes"]])) * f["q"].numel() for f in vol)
-    tpairs = (steps + 4 * sum(f["starts"].shape[0] for f in top)) * 0  # filled below
     tpairs = sum((int(r.steps.sum().item()) + 4 * r.steps.numel()) * f["q"].numel()
                  for r, f in zip(results["top"], top))
     out = {"workload": "md_frames", "frames": n_frames, "volume_frames": len(vol),
