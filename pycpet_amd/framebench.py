@@ -161,7 +161,7 @@ def run_serve(rate: float, duration: float, lanes: int, seed: int = 0,
     warm = serve.poisson_requests(50.0, 0.2, seed=seed + 1, max_requests=6)
     srv = serve.Server(n_lanes=lanes, hbm_budget=64 << 30)
     srv.run(warm)
-    st = srv.run(reqs).summary()
+    st = srv.run(reqs, progress=lambda m: print(m, file=sys.stderr, flush=True)).summary()
     print(f"continuous batching: {st}", file=sys.stderr, flush=True)
     for r in reqs:
         r.result = None

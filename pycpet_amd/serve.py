@@ -162,9 +162,11 @@ class Server:
         return n
 
     def run(self, requests: List[Request], stats: Optional[ServeStats] = None,
-            poll_s: float = 2e-4) -> ServeStats:
-        """Serves `requests` (sorted by arrival); returns when all are done."""
+            poll_s: float = 2e-4, progress=None) -> ServeStats:
+        """Serves `requests` (sorted by arrival); returns when all are done.
+        `progress`, if given, receives a status line every 500 completions."""
         stats = stats or ServeStats()
+        next_report = 500
         arrivals = collections.deque(sorted(requests, key=lambda r: r.arrival))
         pending: collections.deque = collections.deque()
         t0 = time.perf_counter()
@@ -181,6 +183,9 @@ class Server:
                 pending.append(r)
             for lane in self.lanes:
                 remaining -= self._finish_ready(lane, t0, stats)
+            if progress is not None and stats.completed >= next_report:
+                progress(f"batched: {stats.completed} done at {now:.1f} s, {len(pending)} queued")
+                next_report += 500
             for lane in self.lanes:
                 if lane.busy or not pending:
                     continue
