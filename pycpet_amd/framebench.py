@@ -152,26 +152,28 @@ def run_batch(n_frames: int, topo_starts: int, seed: int = 3) -> dict:
 
 
 def run_serve(rate: float, duration: float, lanes: int, seed: int = 0,
-              plain_max_s: float = 120.0) -> dict:
+              plain_max_s: float = 120.0, policy: str = "fifo-small") -> dict:
     import torch
 
     from . import serve
 
     reqs = serve.poisson_requests(rate, duration, seed=seed)
     warm = serve.poisson_requests(50.0, 0.2, seed=seed + 1, max_requests=6)
-    srv = serve.Server(n_lanes=lanes, hbm_budget=64 << 30)
+    srv = serve.Server(n_lanes=lanes, hbm_budget=64 << 30, policy=policy)
     srv.run(warm)
-    st = srv.run(reqs, progress=lambda m: print(m, file=sys.stderr, flush=True)).summary()
+    st = srv.run(reqs, progress=lambda m: print(m, file=sys.stderr, flush=True),
+                 keep_results=False).summary()
     print(f"continuous batching: {st}", file=sys.stderr, flush=True)
     for r in reqs:
         r.result = None
     torch.cuda.synchronize()
     pl = serve.serve_plain(reqs, max_wall_s=plain_max_s,
-                           progress=lambda m: print(m, file=sys.stderr, flush=True)).summary()
+                           progress=lambda m: print(m, file=sys.stderr, flush=True),
+                           keep_results=False).summary()
     print(f"plain: {pl}", file=sys.stderr, flush=True)
     kinds = {k: sum(r.kind == k for r in reqs) for k in ("volume", "volume_ESP", "topo")}
     return {"workload": "poisson_serving", "rate_req_per_s": rate, "duration_s": duration,
-            "requests": len(reqs), "mix": kinds, "lanes": lanes,
+            "requests": len(reqs), "mix": kinds, "lanes": lanes, "policy": policy,
             "offered_frames_per_s": len(reqs) / duration,
             "continuous_batching": st, "plain_one_at_a_time": pl,
             "throughput_ratio": st["frames_per_s"] / max(pl["frames_per_s"], 1e-9),
@@ -188,13 +190,14 @@ def main(argv=None) -> int:
     s.add_argument("--rate", type=float, default=200.0)
     s.add_argument("--duration", type=float, default=10.0)
     s.add_argument("--lanes", type=int, default=4)
+    s.add_argument("--policy", default="fifo-small", choices=("fifo-small", "greedy"))
     s.add_argument("--plain-max-s", type=float, default=120.0,
                    help="stop the one-at-a-time baseline after this wall time")
     a = ap.parse_args(argv)
     if a.cmd == "batch":
         out = run_batch(a.frames, a.topo_starts)
     else:
-        out = run_serve(a.rate, a.duration, a.lanes, plain_max_s=a.plain_max_s)
+        out = run_serve(a.rate, a.duration, a.lanes, plain_max_s=a.plain_max_s, policy=a.policy)
     print(json.dumps(out), flush=True)
     return 0
 

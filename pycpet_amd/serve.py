@@ -1,16 +1,27 @@
 """Request serving with continuous batching on one GPU.
 
 Requests (volume / volume_ESP / topo frames) arrive over time.  The server
-keeps `n_lanes` HIP streams (each with its own libcpet_hip context).  Every
-time a lane is free it takes all admitted requests that fit the HBM budget
-(and `max_batch`), copies their inputs to the device, and launches one batch
-call per request kind (cpet_batch_volume / cpet_batch_topo), each followed
-by asynchronous copies of its results into pinned host memory and an event:
-a kind's requests complete when their own event fires.  Requests that
-arrive while other batches run are therefore admitted into the next free
-lane and their work overlaps on the device with the running batches.
-Latency is measured from a request's arrival to its results being in host
-memory.
+keeps `n_lanes` HIP streams (each with its own libcpet_hip context).  A lane
+that is free takes admitted requests that fit the HBM budget, copies their
+inputs to the device, and launches one batch call per request kind
+(cpet_batch_volume / cpet_batch_topo), each followed by asynchronous copies
+of its results into pinned host memory and an event: a kind's requests
+complete when their own event fires.  Latency is measured from a request's
+arrival to its results being in host memory.
+
+Policy "fifo-small" (the default) sorts requests by estimated work:
+  * a large request (>= small_work pair evaluations) fills the GPU by
+    itself.  Large requests run in arrival order on lane 0 through the
+    per-frame entry points (cpet_field_grid / cpet_topo, whose charge splits
+    suit one big frame), one at a time on the device; the next one is
+    enqueued while the current one runs, so its input copies overlap.
+    Batching them would only make the first of a batch wait for the rest;
+  * small requests cannot fill the GPU alone: a free lane among the others
+    coalesces every queued small request (up to max_batch) into one batch
+    call per kind, and these run beside the large request in flight instead
+    of queueing behind it.
+Policy "greedy" lets every free lane take every queued request (up to
+max_batch) into batch calls, whatever its size.
 
 The "plain" mode serves the same arrivals one request at a time on one
 stream through the per-frame entry points (cpet_field_grid / cpet_topo),
@@ -44,9 +55,29 @@ class Request:
     t_done: float = math.nan
     result: object = None
     error: Optional[str] = None
+    _work: Optional[float] = None
+    _nbytes: Optional[int] = None
+
+    def work(self) -> float:
+        """Pair evaluations, an upper bound for streamlines (every start
+        running its n_iter steps); computed once."""
+        if self._work is None:
+            d = self.data
+            n = float(d["q"].size)
+            if self.kind in ("volume", "volume_ESP"):
+                self._work = n * float(np.prod([len(a) for a in d["axes"]]))
+            else:
+                self._work = n * float(np.sum(d["n_iter"]) + 4 * d["starts"].shape[0])
+        return self._work
 
     def nbytes(self) -> int:
-        """Device footprint: inputs, outputs and per-request scratch."""
+        """Device footprint: inputs, outputs and per-request scratch;
+        computed once."""
+        if self._nbytes is None:
+            self._nbytes = self._footprint()
+        return self._nbytes
+
+    def _footprint(self) -> int:
         d = self.data
         n = int(d["q"].size)
         b = 16 * n + 32 * n  # raw + packed charges (float4 and double4)
@@ -90,16 +121,37 @@ class _Lane:
         self.torch = torch
         self.stream = torch.cuda.Stream(device=device)  # default priority
         self.ctx = nat.Context(device)
-        self.groups = collections.deque()  # (event, [(request, pinned result)])
+        self.groups = collections.deque()  # (event, [(request, (pinned view, pinned buffer))])
 
     @property
     def busy(self) -> bool:
         return bool(self.groups)
 
 
+class _PinnedPool:
+    """Reusable pinned host staging buffers in power-of-two size classes.
+    A result is copied device -> pinned asynchronously, then into pageable
+    memory when its request completes, and the pinned buffer is reused; so
+    the pinned memory held is bounded by the results in flight, not by how
+    many requests a long run has served."""
+
+    def __init__(self, torch):
+        self.torch = torch
+        self.free = collections.defaultdict(list)
+
+    def get(self, nbytes: int):
+        size = max(4096, 1 << max(0, int(nbytes - 1).bit_length()))
+        if self.free[size]:
+            return self.free[size].pop()
+        return self.torch.empty((size,), dtype=self.torch.uint8, pin_memory=True)
+
+    def put(self, buf) -> None:
+        self.free[buf.numel()].append(buf)
+
+
 class Server:
     def __init__(self, device: int = 0, n_lanes: int = 4, hbm_budget: int = 8 << 30,
-                 max_batch: int = 64):
+                 max_batch: int = 64, policy: str = "fifo-small", small_work: float = 2e9):
         import torch
 
         self.torch = torch
@@ -108,24 +160,75 @@ class Server:
         self.budget = hbm_budget
         self.max_batch = max_batch
         self.inflight = 0
+        self.pool = _PinnedPool(torch)
+        self.keep_results = True
+        if policy not in ("fifo-small", "greedy"):
+            raise ValueError(f"unknown policy {policy!r}")
+        self.policy = policy
+        self.small_work = small_work
+
+    def _take(self, lane_idx: int, small: collections.deque, large: collections.deque) -> tuple:
+        """The requests lane `lane_idx` takes next from the FIFO queues of
+        small and large requests, and whether they are one large request
+        (see the module docstring).  Under "greedy" every request is queued
+        as small."""
+        lane = self.lanes[lane_idx]
+        if self.policy == "fifo-small" and lane_idx == 0 and len(self.lanes) > 1:
+            if large and len(lane.groups) < 2 and \
+                    self.inflight + large[0].nbytes() <= self.budget:
+                return [large.popleft()], True
+            return [], False
+        if lane.busy:
+            return [], False
+        if self.policy == "fifo-small" and len(self.lanes) == 1 and large and \
+                self.inflight + large[0].nbytes() <= self.budget:
+            return [large.popleft()], True
+        take, acc = [], 0
+        while small and len(take) < self.max_batch and \
+                self.inflight + acc + small[0].nbytes() <= self.budget:
+            acc += small[0].nbytes()
+            take.append(small.popleft())
+        return take, False
 
     # -------------------------------------------------------------- batching
+    def _to_dev(self, a):
+        return self.torch.from_numpy(np.ascontiguousarray(a)).to(self.dev, non_blocking=True)
+
+    def _pinned(self, t):
+        """Asynchronous copy of device tensor t into a pooled pinned buffer."""
+        nbytes = t.numel() * t.element_size()
+        buf = self.pool.get(nbytes)
+        h = buf[:nbytes].view(t.dtype).view(t.shape)
+        h.copy_(t, non_blocking=True)
+        return h, buf
+
+    def _close_group(self, lane: _Lane, outputs) -> None:
+        ev = self.torch.cuda.Event()
+        ev.record(lane.stream)
+        lane.groups.append((ev, outputs))
+
+    def _launch_one(self, lane: _Lane, r: Request) -> None:
+        """One large request through the per-frame entry point on the lane's
+        stream, then its result copy and event."""
+        torch, T, d = self.torch, self._to_dev, r.data
+        with torch.cuda.stream(lane.stream):
+            if r.kind == "topo":
+                out = ops.topo(T(d["starts"]), T(d["n_iter"]), d["h"], d["dims"], T(d["x"]),
+                               T(d["q"]), ctx=lane.ctx).feats
+            else:
+                E, phi = ops.field_grid(*(T(a) for a in d["axes"]), T(d["x"]), T(d["q"]),
+                                        want_E=r.kind == "volume",
+                                        want_phi=r.kind == "volume_ESP", ctx=lane.ctx)
+                out = E if r.kind == "volume" else phi
+            self._close_group(lane, [(r, self._pinned(out))])
+
     def _launch(self, lane: _Lane, reqs: List[Request]) -> None:
         """Queues one batch call per request kind on the lane's stream, each
         followed by its results' copies into pinned memory and an event, so
         a kind's requests complete as soon as their own work is done."""
-        torch = self.torch
-        T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(self.dev, non_blocking=True)
-
-        def pinned(t):
-            h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
-            h.copy_(t, non_blocking=True)
-            return h
-
-        def close_group(outputs):
-            ev = torch.cuda.Event()
-            ev.record(lane.stream)
-            lane.groups.append((ev, outputs))
+        torch, T = self.torch, self._to_dev
+        pinned = self._pinned
+        close_group = lambda outputs: self._close_group(lane, outputs)
 
         with torch.cuda.stream(lane.stream):
             for kind in ("volume", "volume_ESP", "topo"):
@@ -152,8 +255,10 @@ class Server:
         while lane.groups and lane.groups[0][0].query():
             _, outputs = lane.groups.popleft()
             now = time.perf_counter() - t0
-            for r, h in outputs:
-                r.result = h.numpy()
+            for r, (h, buf) in outputs:
+                res = h.numpy().copy()
+                r.result = res if self.keep_results else None
+                self.pool.put(buf)
                 r.t_done = now
                 stats.latencies.append(r.t_done - r.arrival)
                 stats.completed += 1
@@ -162,13 +267,17 @@ class Server:
         return n
 
     def run(self, requests: List[Request], stats: Optional[ServeStats] = None,
-            poll_s: float = 2e-4, progress=None) -> ServeStats:
+            poll_s: float = 2e-4, progress=None, keep_results: bool = True) -> ServeStats:
         """Serves `requests` (sorted by arrival); returns when all are done.
-        `progress`, if given, receives a status line every 500 completions."""
+        `progress`, if given, receives a status line every 500 completions.
+        keep_results=False copies each result to the host and drops it (a
+        load test of many requests), instead of storing it in r.result."""
+        self.keep_results = keep_results
         stats = stats or ServeStats()
         next_report = 500
         arrivals = collections.deque(sorted(requests, key=lambda r: r.arrival))
-        pending: collections.deque = collections.deque()
+        small: collections.deque = collections.deque()
+        large: collections.deque = collections.deque()
         t0 = time.perf_counter()
         remaining = len(requests)
         while remaining > 0:
@@ -180,27 +289,31 @@ class Server:
                     stats.rejected += 1
                     remaining -= 1
                     continue
-                pending.append(r)
+                if self.policy == "fifo-small" and r.work() >= self.small_work:
+                    large.append(r)
+                else:
+                    small.append(r)
             for lane in self.lanes:
                 remaining -= self._finish_ready(lane, t0, stats)
             if progress is not None and stats.completed >= next_report:
-                progress(f"batched: {stats.completed} done at {now:.1f} s, {len(pending)} queued")
+                progress(f"batched: {stats.completed} done at {now:.1f} s, "
+                         f"{len(small) + len(large)} queued")
                 next_report += 500
-            for lane in self.lanes:
-                if lane.busy or not pending:
-                    continue
-                take = []
-                while pending and len(take) < self.max_batch and \
-                        self.inflight + pending[0].nbytes() <= self.budget:
-                    r = pending.popleft()
+            for li, lane in enumerate(self.lanes):
+                if not (small or large):
+                    break
+                take, is_large = self._take(li, small, large)
+                for r in take:
                     r.t_admit = time.perf_counter() - t0
                     self.inflight += r.nbytes()
-                    take.append(r)
                 if take:
                     stats.max_inflight_bytes = max(stats.max_inflight_bytes, self.inflight)
-                    self._launch(lane, take)
+                    if is_large:
+                        self._launch_one(lane, take[0])
+                    else:
+                        self._launch(lane, take)
                     stats.batches += 1
-            if not any(l.busy for l in self.lanes) and not pending and arrivals:
+            if not any(l.busy for l in self.lanes) and not (small or large) and arrivals:
                 time.sleep(max(0.0, min(arrivals[0].arrival - now, 0.01)))
             else:
                 time.sleep(poll_s)
@@ -210,13 +323,14 @@ class Server:
 
 def serve_plain(requests: List[Request], device: int = 0,
                 stats: Optional[ServeStats] = None, max_wall_s: Optional[float] = None,
-                progress=None) -> ServeStats:
+                progress=None, keep_results: bool = True) -> ServeStats:
     """One request at a time, per-frame entry points, one stream.
 
     With max_wall_s, serving stops at that wall time; requests not served
     by then are counted in `stats.rejected` and their latency enters the
     percentiles as the lower bound (stop time - arrival).  `progress`, if
-    given, is called with a status string every 100 requests."""
+    given, is called with a status string every 100 requests.
+    keep_results=False copies each result to the host and drops it."""
     import torch
 
     stats = stats or ServeStats()
@@ -248,7 +362,8 @@ def serve_plain(requests: List[Request], device: int = 0,
             res = ops.topo(T(d["starts"]), T(d["n_iter"]), d["h"], d["dims"], T(d["x"]),
                            T(d["q"]), ctx=ctx)
             out = res.feats
-        r.result = out.cpu().numpy()
+        res = out.cpu().numpy()
+        r.result = res if keep_results else None
         r.t_done = time.perf_counter() - t0
         busy[r.kind] += r.t_done - r.t_admit
         stats.latencies.append(r.t_done - r.arrival)

@@ -17,16 +17,24 @@ def _rel(a, b):
     return float((d / np.maximum(nb, 1e-3 * np.sqrt(np.mean(nb ** 2)))).max())
 
 
-def test_alone_equals_in_the_mix(gpu_ctx):
+@pytest.mark.parametrize("policy,small_work", [("fifo-small", 2e9), ("greedy", 2e9),
+                                               ("fifo-small", 0.0)])
+def test_alone_equals_in_the_mix(gpu_ctx, policy, small_work):
+    """fifo-small with the default threshold coalesces these small requests;
+    with small_work = 0 every request is large and runs alone, one at a time."""
     from pycpet_amd import ops, serve
 
     reqs = serve.poisson_requests(rate=400.0, duration=0.2, seed=3, n_range=(500, 6000),
                                   edge_range=(9, 31), starts_range=(200, 3000), max_requests=40)
     assert {r.kind for r in reqs} == {"volume", "volume_ESP", "topo"}
-    srv = serve.Server(n_lanes=3, hbm_budget=1 << 30, max_batch=8)
+    srv = serve.Server(n_lanes=3, hbm_budget=1 << 30, max_batch=8, policy=policy,
+                       small_work=small_work)
     stats = srv.run(reqs)
     assert stats.completed == len(reqs) and stats.rejected == 0
-    assert stats.batches < len(reqs)  # requests were batched together
+    if small_work > 0:
+        assert stats.batches < len(reqs)  # requests were batched together
+    else:
+        assert stats.batches == len(reqs)  # one large request per launch
     for r in reqs:
         d = r.data
         if r.kind == "topo":
