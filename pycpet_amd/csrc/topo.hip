@@ -286,10 +286,11 @@ struct ChargeSrc {
       }
     }, ex, ey, ez);
   }
+  // Sources [g0, g1), both multiples of 16.
   __device__ static void field2_f64(const double (&p)[2][3], double (&e)[2][3],
-                                    const float4* __restrict__ ch, int npad) {
-    constexpr int kGroup = 8;  // one scalar-load wait per 8 charges (npad % 64 == 0)
-    for (int g = 0; g < npad; g += kGroup) {
+                                    const float4* __restrict__ ch, int g0, int g1) {
+    constexpr int kGroup = 8;  // one scalar-load wait per 8 charges
+    for (int g = g0; g < g1; g += kGroup) {
       float4 cg[kGroup];
 #pragma unroll
       for (int u = 0; u < kGroup; ++u) cg[u] = ch[g + u];
@@ -340,10 +341,11 @@ struct DipoleSrc {
       }
     }, ex, ey, ez);
   }
+  // Sources [g0, g1), both multiples of 16.
   __device__ static void field2_f64(const double (&p)[2][3], double (&e)[2][3],
-                                    const float4* __restrict__ ch, int npad) {
+                                    const float4* __restrict__ ch, int g0, int g1) {
     constexpr int kGroup = 4;  // 4 dipoles = 8 float4 per scalar-load wait
-    for (int g = 0; g < npad; g += kGroup) {
+    for (int g = g0; g < g1; g += kGroup) {
       float4 cg[2 * kGroup];
 #pragma unroll
       for (int u = 0; u < 2 * kGroup; ++u) cg[u] = ch[2 * g + u];
@@ -578,14 +580,41 @@ struct EpiArgs {
   int lds_hist;
 };
 
+// Epilogue blocks: kEpiWaves waves share kEpiLanes streamlines (one per
+// lane); wave w sums sources [w npad / kEpiWaves, (w+1) npad / kEpiWaves) and
+// the partial fields are added through LDS in wave order, so every wave
+// holds the same FP64 field.  This gives the launch four times the waves of
+// one lane per streamline, which the FP64 sum's latency needs.
+constexpr int kEpiWaves = 4;
+constexpr int kEpiLanes = 64;
+
+struct EpiShared {
+  double part[kEpiWaves][2][3][kEpiLanes];
+};
+
 // One Euler step for two points at once, FP64 throughout.
 template <class Src>
-__device__ __forceinline__ void step2_f64(const double (&p)[2][3],
-                                          double (&o)[2][3],
-                                          const float4* __restrict__ ch,
-                                          int npad, double h) {
+__device__ __forceinline__ void step2_f64(const double (&p)[2][3], double (&o)[2][3],
+                                          const float4* __restrict__ ch, int npad, double h,
+                                          EpiShared& sh, int wave, int lane) {
+  const int per = npad / kEpiWaves;  // npad is a multiple of 64
   double e[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
-  Src::field2_f64(p, e, ch, npad);
+  Src::field2_f64(p, e, ch, wave * per, wave * per + per);
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) sh.part[wave][k][c][lane] = e[k][c];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double v = 0.0;
+#pragma unroll
+      for (int w = 0; w < kEpiWaves; ++w) v += sh.part[w][k][c][lane];
+      e[k][c] = v;
+    }
+  __syncthreads();  // part is reused by the next step
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     const double ex = kCoulomb * e[k][0], ey = kCoulomb * e[k][1],
@@ -608,19 +637,24 @@ __device__ __forceinline__ double curve_f64(const double (&a)[3],
 }
 
 // Features (and histogram count) of streamline l of one frame.
+// Every wave of the block calls this for its lane's streamline l (l < L
+// or not: the steps have barriers); wave 0 stores the features.
 template <class Src>
 __device__ __forceinline__ void epilogue_one(const float4* __restrict__ ch, int npad,
                                              const float* starts, const double* fin,
-                                             double h, double* feats, int l,
+                                             double h, double* feats, int l, int L,
                                              const EpiArgs& a, unsigned int* lh,
-                                             unsigned int* hist) {
+                                             unsigned int* hist, EpiShared& sh) {
+  const int wave = threadIdx.x / kEpiLanes, lane = threadIdx.x % kEpiLanes;
+  const bool valid = l < L;
   double p[2][3], q1[2][3], q2[2][3];
   for (int i = 0; i < 3; ++i) {
-    p[0][i] = (double)starts[3 * l + i];
-    p[1][i] = fin[3 * l + i];
+    p[0][i] = valid ? (double)starts[3 * l + i] : 0.0;
+    p[1][i] = valid ? fin[3 * l + i] : 0.0;
   }
-  step2_f64<Src>(p, q1, ch, npad, h);
-  step2_f64<Src>(q1, q2, ch, npad, h);
+  step2_f64<Src>(p, q1, ch, npad, h, sh, wave, lane);
+  step2_f64<Src>(q1, q2, ch, npad, h, sh, wave, lane);
+  if (!valid || wave != 0) return;
   double kap[2];
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
@@ -650,16 +684,17 @@ __device__ __forceinline__ void epilogue_one(const float4* __restrict__ ch, int 
 }
 
 template <class Src>
-__global__ __launch_bounds__(256) void topo_epilogue_kernel(
+__global__ __launch_bounds__(kEpiWaves * kEpiLanes) void topo_epilogue_kernel(
     const float4* __restrict__ ch, EpiArgs a) {
   extern __shared__ unsigned int lh[];
+  __shared__ EpiShared sh;
   const int nbins = a.nbd * a.nbc;
   if (a.hist && a.lds_hist) {
     for (int i = threadIdx.x; i < nbins; i += blockDim.x) lh[i] = 0u;
     __syncthreads();
   }
-  const int l = blockIdx.x * blockDim.x + threadIdx.x;
-  if (l < a.L) epilogue_one<Src>(ch, a.npad, a.starts, a.fin, a.h, a.feats, l, a, lh, a.hist);
+  const int l = blockIdx.x * kEpiLanes + threadIdx.x % kEpiLanes;
+  epilogue_one<Src>(ch, a.npad, a.starts, a.fin, a.h, a.feats, l, a.L, a, lh, a.hist, sh);
   if (a.hist && a.lds_hist) {
     __syncthreads();
     for (int i = threadIdx.x; i < nbins; i += blockDim.x)
@@ -670,11 +705,12 @@ __global__ __launch_bounds__(256) void topo_epilogue_kernel(
 // Batch epilogue: blk[2b] = frame, blk[2b+1] = block within the frame, so a
 // workgroup only ever reads one frame's charges.  Histograms are per frame
 // (hist + frame * nbins) over shared edges.
-__global__ __launch_bounds__(256) void topo_batch_epilogue_kernel(
+__global__ __launch_bounds__(kEpiWaves * kEpiLanes) void topo_batch_epilogue_kernel(
     const float4* __restrict__ ch, EpiArgs a, const int32_t* __restrict__ blk,
     const int32_t* __restrict__ chg_off, const int32_t* __restrict__ sl_off,
     const double* __restrict__ hdims) {
   extern __shared__ unsigned int lh[];
+  __shared__ EpiShared sh;
   const int nbins = a.nbd * a.nbc;
   const int fr = blk[2 * blockIdx.x], b = blk[2 * blockIdx.x + 1];
   if (a.hist && a.lds_hist) {
@@ -683,11 +719,11 @@ __global__ __launch_bounds__(256) void topo_batch_epilogue_kernel(
   }
   const int s0 = sl_off[fr];
   const int L = sl_off[fr + 1] - s0;
-  const int l = b * blockDim.x + threadIdx.x;
+  const int l = b * kEpiLanes + threadIdx.x % kEpiLanes;
   unsigned int* hist = a.hist ? a.hist + (size_t)fr * nbins : nullptr;
-  if (l < L)
-    epilogue_one<ChargeSrc>(ch + chg_off[fr], chg_off[fr + 1] - chg_off[fr], a.starts + 3LL * s0,
-                 a.fin + 3LL * s0, hdims[4 * fr], a.feats + 2LL * s0, l, a, lh, hist);
+  epilogue_one<ChargeSrc>(ch + chg_off[fr], chg_off[fr + 1] - chg_off[fr], a.starts + 3LL * s0,
+                          a.fin + 3LL * s0, hdims[4 * fr], a.feats + 2LL * s0, l, L, a, lh,
+                          hist, sh);
   if (a.hist && a.lds_hist) {
     __syncthreads();
     for (int i = threadIdx.x; i < nbins; i += blockDim.x)
@@ -840,13 +876,16 @@ int topo_run(cpet_ctx* ctx, int N, int L, const float* starts,
                        d_hist, (long long)nbins, 0u);
   const size_t lds = ea.lds_hist ? (size_t)nbins * sizeof(unsigned int) : 0;
   if (src == kSrcCharge)
-    hipLaunchKernelGGL(topo_epilogue_kernel<ChargeSrc>, dim3((L + 255) / 256), dim3(256), lds,
+    hipLaunchKernelGGL(topo_epilogue_kernel<ChargeSrc>, dim3((L + kEpiLanes - 1) / kEpiLanes),
+                       dim3(kEpiWaves * kEpiLanes), lds,
                        st, ch, ea);
   else if (src == kSrcDipoleRef)
-    hipLaunchKernelGGL(topo_epilogue_kernel<DipoleSrc<false>>, dim3((L + 255) / 256), dim3(256),
+    hipLaunchKernelGGL(topo_epilogue_kernel<DipoleSrc<false>>, dim3((L + kEpiLanes - 1) / kEpiLanes),
+                       dim3(kEpiWaves * kEpiLanes),
                        lds, st, ch, ea);
   else
-    hipLaunchKernelGGL(topo_epilogue_kernel<DipoleSrc<true>>, dim3((L + 255) / 256), dim3(256),
+    hipLaunchKernelGGL(topo_epilogue_kernel<DipoleSrc<true>>, dim3((L + kEpiLanes - 1) / kEpiLanes),
+                       dim3(kEpiWaves * kEpiLanes),
                        lds, st, ch, ea);
   CPET_HIP_TRY(hipGetLastError());
   if ((rc = prof_mark(ctx, 3))) return rc;
@@ -925,7 +964,7 @@ int topo_batch_run(cpet_ctx* ctx, int F, const int32_t* n_starts, const float* s
     for (int r = 0; r < (n_starts[f] + A - 1) / A; ++r) assign.push_back(f);
   std::vector<int32_t> blk;
   for (int f = 0; f < F; ++f)
-    for (int b = 0; b < (n_starts[f] + 255) / 256; ++b) {
+    for (int b = 0; b < (n_starts[f] + kEpiLanes - 1) / kEpiLanes; ++b) {
       blk.push_back(f);
       blk.push_back(b);
     }
@@ -1038,7 +1077,8 @@ int topo_batch_run(cpet_ctx* ctx, int F, const int32_t* n_starts, const float* s
   ea.hist = d_hist;
   ea.lds_hist = (hist && nbins <= 8192) ? 1 : 0;
   const size_t lds = ea.lds_hist ? (size_t)nbins * sizeof(unsigned int) : 0;
-  hipLaunchKernelGGL(topo_batch_epilogue_kernel, dim3((unsigned)(blk.size() / 2)), dim3(256), lds,
+  hipLaunchKernelGGL(topo_batch_epilogue_kernel, dim3((unsigned)(blk.size() / 2)),
+                     dim3(kEpiWaves * kEpiLanes), lds,
                      st, ch, ea, (const int32_t*)(tw + o_blk), d_chg_off,
                      (const int32_t*)(tw + o_sl), (const double*)(tw + o_hd));
   CPET_HIP_TRY(hipGetLastError());
