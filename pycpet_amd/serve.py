@@ -152,6 +152,8 @@ class _PinnedPool:
 class Server:
     def __init__(self, device: int = 0, n_lanes: int = 4, hbm_budget: int = 8 << 30,
                  max_batch: int = 64, policy: str = "fifo-small", small_work: float = 2e9):
+        if policy not in ("fifo-small", "greedy"):
+            raise ValueError(f"unknown policy {policy!r}")
         import torch
 
         self.torch = torch
@@ -162,8 +164,6 @@ class Server:
         self.inflight = 0
         self.pool = _PinnedPool(torch)
         self.keep_results = True
-        if policy not in ("fifo-small", "greedy"):
-            raise ValueError(f"unknown policy {policy!r}")
         self.policy = policy
         self.small_work = small_work
 
