@@ -22,9 +22,13 @@
 //      a fixed order through LDS and every wave applies the identical FP64
 //      position update.  A finished slot takes the next id from a global
 //      queue.  Results do not depend on which slot runs a streamline.
-//   3. epilogue: one lane per streamline, the four curvature-defining steps
-//      in FP64 (FP32 finite differences at h = 1e-3 lose most digits),
-//      distance, and the 2-D histogram with numpy.histogramdd bin rules.
+//   3. epilogue: one lane per streamline in each of four waves that split
+//      its sources, the four curvature-defining steps in FP64 (FP32 finite
+//      differences at h = 1e-3 lose most digits), distance, and the 2-D
+//      histogram with numpy.histogramdd bin rules.
+//   Batches run the same three stages for many frames: a per-frame
+//   longest-first order, one persistent launch whose workgroups take
+//   frames by assignment, and a per-frame epilogue.
 
 #include <algorithm>
 #include <cmath>
