@@ -8,6 +8,12 @@ pick an unfinished input at random, skip it when its output already exists
 `calculator`, compute, and write <frame>.top / <frame>_efield.dat /
 <frame>_esp.dat / point_field.dat in the reference formats.  Methods outside
 the hot path (cluster, pca, visualize, box_check) raise NotImplementedError.
+
+"topo" / "topo_GPU" run up to `frames_per_batch` unfinished frames (an
+option of this build, default 16; 1 gives the reference's one frame per
+call) through one batched streamline launch and then write their .top
+files.  The batch computes each frame exactly as the per-frame call does, so
+the files are the same; frames that alone cannot fill the GPU run together.
 """
 from __future__ import annotations
 
@@ -69,9 +75,31 @@ class CPET:
 
     def run_topo(self, gpu=False):
         runtype = "compute_topo_GPU_batch_filter" if gpu else "compute_topo_complete_c_shared"
+        per = max(1, int(self.options.get("frames_per_batch", 16)))
+        group = []
         for f, frame in self._frames(".top"):
-            self.calculator = calculator(self.options, path_to_pdb=f)
-            hist = getattr(self.calculator, runtype)()
+            group.append((frame, calculator(self.options, path_to_pdb=f)))
+            if len(group) == per:
+                self._topo_group(group, runtype)
+                group = []
+        if group:
+            self._topo_group(group, runtype)
+
+    def _topo_group(self, group, runtype):
+        """Streamline features of the frames of `group` ((frame, calculator)
+        pairs), one batched launch for several, then their .top files."""
+        if len(group) > 1:
+            from .batch import batch_topo
+
+            results = batch_topo([{"x": c.x, "q": c.Q, "starts": c.random_start_points,
+                                   "n_iter": np.asarray(c.random_max_samples),
+                                   "h": c.step_size, "dims": c.dimensions} for _, c in group])
+            for (_, c), r in zip(group, results):
+                c.topo_result = r
+                c.hist = r.feats.astype(np.float32)
+        for frame, c in group:
+            self.calculator = c
+            hist = c.hist if len(group) > 1 else getattr(c, runtype)()
             write_rows(self.outputpath + f"/{frame}.top", hist, "%.18e")
 
     def _meta(self, mesh_shape):

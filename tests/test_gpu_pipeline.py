@@ -180,3 +180,23 @@ def test_end_to_end_options_file_methods(tmp_path, gpu_ctx, oracle_mod):
         top = np.loadtxt(tmp_path / "out" / "frame1.top")
         assert top.shape == (216, 2)
         (tmp_path / "out" / "frame1.top").unlink()
+
+
+def test_topo_frames_batched_write_the_same_files(tmp_path, gpu_ctx):
+    """CPET topo with several frames per batched launch writes byte-identical
+    .top files to one frame per call (the reference's loop)."""
+    from pycpet_amd.CPET import CPET
+
+    (tmp_path / "in").mkdir()
+    for i in range(3):
+        synthetic_frame(tmp_path / "in" / f"frame{i}.pdb", n=300 + 100 * i, seed=10 + i)
+    texts = {}
+    for per in (1, 3):
+        out = tmp_path / f"out{per}"
+        opts = _options(tmp_path, "topo", n_samples=120, step_size=0.01,
+                        max_streamline_init="fixed_rand", dimensions=[0.5, 0.5, 0.5],
+                        frames_per_batch=per)
+        opts["outputpath"] = str(out)
+        CPET(opts).run()
+        texts[per] = {p.name: p.read_bytes() for p in sorted(out.glob("*.top"))}
+    assert len(texts[1]) == 3 and texts[1] == texts[3]
