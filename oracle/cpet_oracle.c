@@ -145,6 +145,50 @@ static void step_f64(field_fn fld, int opt, const double p[3], double out[3], in
     out[2] = p[2] + h * E[2] / n;
 }
 
+/* One classical RK4 step (weights 1, 2, 2, 1 over 6) of dx/ds = E/|E|, the
+ * reference's unit-direction equation with its Euler step replaced (an
+ * extension: the reference integrates with Euler only).  The weighted sum
+ * is accumulated u1, +2 u2, +2 u3, +u4, then scaled by h/6. */
+static void unit_dir(field_fn fld, int opt, const double p[3], double u[3], int N,
+                     const float *x, const float *src)
+{
+    double E[3];
+    fld(p, N, x, src, opt, E);
+    const double n = sqrt(E[0] * E[0] + E[1] * E[1] + E[2] * E[2]);
+    u[0] = E[0] / n;
+    u[1] = E[1] / n;
+    u[2] = E[2] / n;
+}
+
+static void step_rk4_f64(field_fn fld, int opt, const double p[3], double out[3], int N,
+                         const float *x, const float *src, double h)
+{
+    double u[3], q[3], acc[3];
+    unit_dir(fld, opt, p, u, N, x, src);
+    for (int i = 0; i < 3; ++i) {
+        acc[i] = u[i];
+        q[i] = p[i] + 0.5 * h * u[i];
+    }
+    unit_dir(fld, opt, q, u, N, x, src);
+    for (int i = 0; i < 3; ++i) {
+        acc[i] += 2.0 * u[i];
+        q[i] = p[i] + 0.5 * h * u[i];
+    }
+    unit_dir(fld, opt, q, u, N, x, src);
+    for (int i = 0; i < 3; ++i) {
+        acc[i] += 2.0 * u[i];
+        q[i] = p[i] + h * u[i];
+    }
+    unit_dir(fld, opt, q, u, N, x, src);
+    for (int i = 0; i < 3; ++i) {
+        acc[i] += u[i];
+        out[i] = p[i] + (h / 6.0) * acc[i];
+    }
+}
+
+typedef void (*step_fn)(field_fn fld, int opt, const double p[3], double out[3], int N,
+                        const float *x, const float *src, double h);
+
 /* |v' x v''| / |v'|^3 (math_module.c:71-121). */
 static double curve_f64(const double a[3], const double b[3])
 {
@@ -161,10 +205,10 @@ static double curve_f64(const double a[3], const double b[3])
  *   steps[l]    = number of Euler steps taken (<= n_iter[l])
  *   exited[l]   = 1 if the walk ended on a box exit, 0 on the step limit
  *   pts6 (opt.) = the six points x0, x0+, x0++, xf, xf+, xf++ (L x 6 x 3)  */
-static void walk_f64(field_fn fld, int opt, int N, int L, const float *starts,
-                     const int32_t *n_iter, float h32, const float *dims, const float *x,
-                     const float *q, double *feats, int32_t *steps, int32_t *exited,
-                     double *pts6)
+static void walk_f64(field_fn fld, int opt, step_fn step_f64, int N, int L,
+                     const float *starts, const int32_t *n_iter, float h32, const float *dims,
+                     const float *x, const float *q, double *feats, int32_t *steps,
+                     int32_t *exited, double *pts6)
 {
     const double h = (double)h32;
     const double d0 = dims[0], d1 = dims[1], d2 = dims[2];
@@ -221,7 +265,19 @@ void oracle_topo_f64(int N, int L, const float *starts, const int32_t *n_iter,
                      const float *q, double *feats, int32_t *steps,
                      int32_t *exited, double *pts6)
 {
-    walk_f64(charge_field, 0, N, L, starts, n_iter, h32, dims, x, q, feats, steps, exited, pts6);
+    walk_f64(charge_field, 0, step_f64, N, L, starts, n_iter, h32, dims, x, q, feats, steps,
+             exited, pts6);
+}
+
+/* The same walk and features with RK4 steps (step_rk4_f64), for the main
+ * walk and the four curvature-defining steps alike. */
+void oracle_topo_rk4_f64(int N, int L, const float *starts, const int32_t *n_iter,
+                         float h32, const float *dims, const float *x,
+                         const float *q, double *feats, int32_t *steps,
+                         int32_t *exited, double *pts6)
+{
+    walk_f64(charge_field, 0, step_rk4_f64, N, L, starts, n_iter, h32, dims, x, q, feats, steps,
+             exited, pts6);
 }
 
 /* Dipole streamlines (thread_operation_dipole, math_module.c:593-659): the
@@ -231,8 +287,8 @@ void oracle_topo_dipole_f64(int N, int L, const float *starts, const int32_t *n_
                             float h32, const float *dims, const float *x, const float *mu,
                             int physical, double *feats, int32_t *steps, int32_t *exited)
 {
-    walk_f64(dipole_field, physical, N, L, starts, n_iter, h32, dims, x, mu, feats, steps,
-             exited, NULL);
+    walk_f64(dipole_field, physical, step_f64, N, L, starts, n_iter, h32, dims, x, mu, feats,
+             steps, exited, NULL);
 }
 
 /* ------------------------------------------------------------------ FP32 */

@@ -52,8 +52,9 @@ def lib():
         L.oracle_topo_f64.argtypes = [_i, _i, _P, _P, _f, _P, _P, _P, _P, _P, _P, _P]
         L.oracle_topo_f32.argtypes = [_i, _i, _P, _P, _f, _P, _P, _P, _P, _P]
         L.oracle_topo_dipole_f64.argtypes = [_i, _i, _P, _P, _f, _P, _P, _P, _i, _P, _P, _P]
+        L.oracle_topo_rk4_f64.argtypes = [_i, _i, _P, _P, _f, _P, _P, _P, _P, _P, _P, _P]
         for fn in (L.oracle_field_f64, L.oracle_field_f32, L.oracle_topo_f64, L.oracle_topo_f32,
-                   L.oracle_topo_dipole_f64):
+                   L.oracle_topo_dipole_f64, L.oracle_topo_rk4_f64):
             fn.restype = None
         _lib = L
     return _lib
@@ -107,6 +108,26 @@ def topo_f64(starts, n_iter, h, dims, x, q, *, want_points=False):
     pts6 = np.zeros((L, 6, 3)) if want_points else None
     lib().oracle_topo_f64(N, L, _p(starts), _p(n_iter), float(np.float32(h)), _p(dims),
                           _p(x), _p(q), _p(feats), _p(steps), _p(exited), _p(pts6))
+    if want_points:
+        return feats, steps, exited, pts6
+    return feats, steps, exited
+
+
+def topo_rk4_f64(starts, n_iter, h, dims, x, q, *, want_points=False):
+    """FP64 streamlines of topo_f64 with classical RK4 steps on the unit
+    field direction (an extension; the reference integrates with Euler).
+    Pinned by: tests/test_oracle_topo.py (straight radial lines equal to the
+    Euler walk, fourth-order convergence of the end point in a curved field,
+    agreement with topo_f64 as h -> 0)."""
+    starts, x, q, dims = _f32(starts, 3), _f32(x, 3), _f32(q), _f32(dims)
+    n_iter = np.ascontiguousarray(n_iter, dtype=np.int32).ravel()
+    L, N = starts.shape[0], q.size
+    feats = np.zeros((L, 2))
+    steps = np.zeros(L, np.int32)
+    exited = np.zeros(L, np.int32)
+    pts6 = np.zeros((L, 6, 3)) if want_points else None
+    lib().oracle_topo_rk4_f64(N, L, _p(starts), _p(n_iter), float(np.float32(h)), _p(dims),
+                              _p(x), _p(q), _p(feats), _p(steps), _p(exited), _p(pts6))
     if want_points:
         return feats, steps, exited, pts6
     return feats, steps, exited

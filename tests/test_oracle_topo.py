@@ -199,3 +199,64 @@ def test_dipole_oracle_matches_golden(oracle_mod):
     want = g["feats"].astype(float)
     close = np.abs(feats[:, 0] - want[:, 0]) <= 1e-3 * np.maximum(want[:, 0], 1e-2)
     assert close.mean() >= 0.9, close.mean()
+
+
+def test_rk4_radial_lines_equal_euler(oracle_mod):
+    """A single charge's field lines are straight radial lines: every RK4
+    stage sees the same direction, so the RK4 walk equals the Euler walk
+    (same exit steps, distance = steps * h, zero curvature)."""
+    x = np.array([[0.0, 0.0, 0.0]], np.float32)
+    q = np.array([1.0], np.float32)
+    rng = np.random.default_rng(2)
+    d = rng.standard_normal((40, 3))
+    starts = (0.3 * d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    n_iter = rng.integers(1, 400, 40).astype(np.int32)
+    dims = np.full(3, 1.0, np.float32)
+    fe, se, ee = oracle_mod.topo_f64(starts, n_iter, 0.01, dims, x, q)
+    fr, sr, er = oracle_mod.topo_rk4_f64(starts, n_iter, 0.01, dims, x, q)
+    np.testing.assert_array_equal(sr, se)
+    np.testing.assert_array_equal(er, ee)
+    np.testing.assert_allclose(fr[:, 0], fe[:, 0], rtol=1e-12)
+    np.testing.assert_allclose(fr[:, 0], sr * float(np.float32(0.01)), rtol=1e-9)
+    assert np.abs(fr[:, 1]).max() < 1e-6
+
+
+def _two_charges():
+    x = np.array([[1.2, 0.3, -0.2], [-1.1, -0.4, 0.5]], np.float32)
+    q = np.array([0.8, -0.6], np.float32)
+    return x, q
+
+
+def test_rk4_fourth_order(oracle_mod):
+    """End point after a fixed arc length in a curved two-charge field: the
+    error against a much finer RK4 walk falls 16x per halving of h (Euler:
+    2x).  A wrong stage weight or stage point lowers the order."""
+    x, q = _two_charges()
+    start = np.array([[0.05, 0.02, 0.0]], np.float32)
+    dims = np.full(3, 50.0, np.float32)  # no exit
+    S = 0.64  # arc length, a multiple of every h below (exact in float32)
+    end = {}
+    for h in (0.08, 0.04, 0.02, 0.0025):
+        n = np.array([int(round(S / h))], np.int32)
+        *_, pts = oracle_mod.topo_rk4_f64(start, n, h, dims, x, q, want_points=True)
+        end[h] = pts[0, 3]
+    e1 = np.linalg.norm(end[0.08] - end[0.0025])
+    e2 = np.linalg.norm(end[0.04] - end[0.0025])
+    e3 = np.linalg.norm(end[0.02] - end[0.0025])
+    assert 12.0 < e1 / e2 < 20.0, e1 / e2
+    assert 12.0 < e2 / e3 < 20.0, e2 / e3
+    # the Euler walk of the same arc is first order and far less accurate
+    *_, pe = oracle_mod.topo_f64(start, np.array([8], np.int32), 0.08, dims, x, q,
+                                 want_points=True)
+    assert np.linalg.norm(pe[0, 3] - end[0.0025]) > 100 * e1
+
+
+def test_rk4_agrees_with_euler_as_h_shrinks(oracle_mod):
+    x, q = _two_charges()
+    start = np.array([[0.05, 0.02, 0.0]], np.float32)
+    dims = np.full(3, 50.0, np.float32)
+    *_, pr = oracle_mod.topo_rk4_f64(start, np.array([20], np.int32), 0.01, dims, x, q,
+                                     want_points=True)
+    *_, pe = oracle_mod.topo_f64(start, np.array([20000], np.int32), 0.00001, dims, x, q,
+                                 want_points=True)
+    assert np.linalg.norm(pr[0, 3] - pe[0, 3]) < 1e-5
