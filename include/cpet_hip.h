@@ -42,6 +42,9 @@ extern "C" {
 #define CPET_F_OUT_F64 0x2u  /* field/phi outputs are double, not float */
 #define CPET_F_NO_SYNC 0x4u  /* device-pointer calls: do not wait for the stream */
 #define CPET_F_DIPOLE_PHYSICAL 0x10u /* cpet_topo_dipole: physical 3(mu.R)R - mu r^2 term */
+#define CPET_F_TOPO_RK4 0x20u  /* cpet_topo: classical RK4 steps on E/|E| for the walk and the
+                                  curvature steps; no counterpart in the reference, which
+                                  integrates with Euler (math_module.c:489-503) */
 /* precision of the field kernels, bits 8..10 */
 #define CPET_F_PREC_AUTO (0u << 8)  /* E: FP32 tiled, points with strong
                                        cancellation recomputed as MIXED;
@@ -152,7 +155,7 @@ int cpet_field_grid(cpet_ctx *ctx, int nx, int ny, int nz, const float *ax,
  * topo_GPU path CPET/source/calculator.py:793-978 computes the same).
  *   starts (L,3) float, n_iter (L) int32, h, dims (3) float
  *   feats  (L,2) double out: [distance, mean curvature] per start, input order
- *   steps  (L) int32 out or NULL: Euler steps taken
+ *   steps  (L) int32 out or NULL: steps taken (Euler, or RK4 with CPET_F_TOPO_RK4)
  *   exited (L) int32 out or NULL: 1 if stopped by the box, 0 by n_iter
  *   hist   (nb_d*nb_c) uint32 out or NULL: 2-D histogram of the float32-
  *          rounded features with numpy.histogramdd edge rules over the

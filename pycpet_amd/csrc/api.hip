@@ -615,7 +615,8 @@ int cpet_topo(cpet_ctx* ctx, int N, int L, const float* starts,
   rc = pack_charges(ctx, N, x, q, dev, &ch, nullptr, &npad);
   if (rc) return rc;
   return topo_run(ctx, N, L, starts, n_iter, h, hdims, ch, npad, feats, steps,
-                  exited, nb_d, edges_d, nb_c, edges_c, hist, flags, kSrcCharge);
+                  exited, nb_d, edges_d, nb_c, edges_c, hist, flags,
+                  (flags & CPET_F_TOPO_RK4) ? kSrcChargeRk4 : kSrcCharge);
 }
 
 int cpet_topo_dipole(cpet_ctx* ctx, int N, int L, const float* starts, const int32_t* n_iter,

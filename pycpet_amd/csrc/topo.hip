@@ -375,6 +375,21 @@ struct DipoleSrc {
   }
 };
 
+// Classical RK4 (weights 1, 2, 2, 1 over 6) on dx/ds = E/|E| with the
+// sources of Base: an extension, selected by CPET_F_TOPO_RK4; the
+// reference integrates with Euler.  Each step evaluates the field four
+// times; positions, the stage points and the weighted direction sum are FP64.
+template <class Base>
+struct Rk4 : Base {};
+template <class Src>
+struct IsRk4 {
+  static constexpr bool value = false;
+};
+template <class Base>
+struct IsRk4<Rk4<Base>> {
+  static constexpr bool value = true;
+};
+
 enum SlotState : int { kActive = 0, kNeed = 1, kDone = 2 };
 
 // One frame's streamline set as the slot loop sees it.  Streamline indices
@@ -402,8 +417,12 @@ template <int S, class Src>
 __device__ __forceinline__ void run_slots(const float4* __restrict__ ch, const FrameView& f,
                                           int first_q, double (*part)[S][3][64],
                                           int (*nsid)[64], int lane, int wave) {
+  constexpr bool rk4 = IsRk4<Src>::value;
   int st = kDone, sid = 0, step = 0, nmax = 0;
   double px = 0.0, py = 0.0, pz = 0.0;
+  // RK4: the stage (0-3), its evaluation point and the weighted direction sum
+  int stage = 0;
+  double qx = 0.0, qy = 0.0, qz = 0.0, ax = 0.0, ay = 0.0, az = 0.0;
 
   auto finish = [&](int out) {
     if (wave == 0) {
@@ -426,6 +445,10 @@ __device__ __forceinline__ void run_slots(const float4* __restrict__ ch, const F
     pz = (double)f.starts[3 * sid + 2];
     nmax = f.n_iter[sid];
     step = 0;
+    stage = 0;
+    qx = px;
+    qy = py;
+    qz = pz;
     if (nmax <= 0)
       finish(0);  // the reference loop body never runs: xf = x0
     else
@@ -440,7 +463,8 @@ __device__ __forceinline__ void run_slots(const float4* __restrict__ ch, const F
   while (__ballot(st != kDone) != 0ull) {
     double fx = 0.0, fy = 0.0, fz = 0.0;
     if (st == kActive)
-      Src::slice((float)px, (float)py, (float)pz, ch, f.t0, f.t1, fx, fy, fz);
+      Src::slice((float)(rk4 ? qx : px), (float)(rk4 ? qy : py), (float)(rk4 ? qz : pz), ch,
+                 f.t0, f.t1, fx, fy, fz);
     part[par][wave][0][lane] = fx;
     part[par][wave][1][lane] = fy;
     part[par][wave][2][lane] = fz;
@@ -458,18 +482,53 @@ __device__ __forceinline__ void run_slots(const float4* __restrict__ ch, const F
       ex *= kCoulomb;
       ey *= kCoulomb;
       ez *= kCoulomb;
-      // x += h E/|E|, with one reciprocal square root instead of an FP64
-      // square root and three divides.
-      const double hn = f.h * rsqrt_f64(fma(ez, ez, fma(ey, ey, ex * ex)));
-      px = fma(hn, ex, px);
-      py = fma(hn, ey, py);
-      pz = fma(hn, ez, pz);
-      ++step;
-      const int out = (px < -f.d0 || px > f.d0 || py < -f.d1 || py > f.d1 ||
-                       pz < -f.d2 || pz > f.d2)
-                          ? 1
-                          : 0;
-      if (out || step >= nmax) finish(out);
+      bool stepped = true;  // a whole step ended (RK4: after its fourth stage)
+      if constexpr (rk4) {
+        const double inv = rsqrt_f64(fma(ez, ez, fma(ey, ey, ex * ex)));
+        const double ux = ex * inv, uy = ey * inv, uz = ez * inv;
+        if (stage == 0) {
+          ax = ux;
+          ay = uy;
+          az = uz;
+        } else {
+          const double w = stage == 3 ? 1.0 : 2.0;
+          ax = fma(w, ux, ax);
+          ay = fma(w, uy, ay);
+          az = fma(w, uz, az);
+        }
+        if (stage < 3) {  // next stage point: x + h/2 u1, x + h/2 u2, x + h u3
+          const double hs = stage == 2 ? f.h : 0.5 * f.h;
+          qx = fma(hs, ux, px);
+          qy = fma(hs, uy, py);
+          qz = fma(hs, uz, pz);
+          ++stage;
+          stepped = false;
+        } else {
+          const double h6 = f.h / 6.0;
+          px = fma(h6, ax, px);
+          py = fma(h6, ay, py);
+          pz = fma(h6, az, pz);
+          qx = px;
+          qy = py;
+          qz = pz;
+          stage = 0;
+        }
+      } else {
+        // x += h E/|E|, with one reciprocal square root instead of an FP64
+        // square root and three divides.
+        const double hn = f.h * rsqrt_f64(fma(ez, ez, fma(ey, ey, ex * ex)));
+        px = fma(hn, ex, px);
+        py = fma(hn, ey, py);
+        pz = fma(hn, ez, pz);
+      }
+      if (stepped) {
+        ++step;
+        const int out = (px < -f.d0 || px > f.d0 || py < -f.d1 || py > f.d1 ||
+                         pz < -f.d2 || pz > f.d2)
+                            ? 1
+                            : 0;
+        if (out || step >= nmax) finish(out);
+      }
     } else if (st == kNeed) {
       load(nsid[par][lane]);
     }
@@ -596,13 +655,18 @@ struct EpiShared {
   double part[kEpiWaves][2][3][kEpiLanes];
 };
 
-// One Euler step for two points at once, FP64 throughout.
+// The unsoftened field (unscaled by k) of all sources at two points, FP64:
+// wave w of the block sums sources [w npad / kEpiWaves, ...) and the partial
+// fields are added through LDS in wave order.
 template <class Src>
-__device__ __forceinline__ void step2_f64(const double (&p)[2][3], double (&o)[2][3],
-                                          const float4* __restrict__ ch, int npad, double h,
-                                          EpiShared& sh, int wave, int lane) {
+__device__ __forceinline__ void field2_split(const double (&p)[2][3], double (&e)[2][3],
+                                             const float4* __restrict__ ch, int npad,
+                                             EpiShared& sh, int wave, int lane) {
   const int per = npad / kEpiWaves;  // npad is a multiple of 64
-  double e[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) e[k][c] = 0.0;
   Src::field2_f64(p, e, ch, wave * per, wave * per + per);
 #pragma unroll
   for (int k = 0; k < 2; ++k)
@@ -618,15 +682,76 @@ __device__ __forceinline__ void step2_f64(const double (&p)[2][3], double (&o)[2
       for (int w = 0; w < kEpiWaves; ++w) v += sh.part[w][k][c][lane];
       e[k][c] = v;
     }
-  __syncthreads();  // part is reused by the next step
+  __syncthreads();  // part is reused by the next evaluation
+}
+
+// Unit field direction at two points, FP64.
+template <class Src>
+__device__ __forceinline__ void unit2_f64(const double (&p)[2][3], double (&u)[2][3],
+                                          const float4* __restrict__ ch, int npad,
+                                          EpiShared& sh, int wave, int lane) {
+  double e[2][3];
+  field2_split<Src>(p, e, ch, npad, sh, wave, lane);
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     const double ex = kCoulomb * e[k][0], ey = kCoulomb * e[k][1],
                  ez = kCoulomb * e[k][2];
     const double n = sqrt(ex * ex + ey * ey + ez * ez);
-    o[k][0] = p[k][0] + h * ex / n;
-    o[k][1] = p[k][1] + h * ey / n;
-    o[k][2] = p[k][2] + h * ez / n;
+    u[k][0] = ex / n;
+    u[k][1] = ey / n;
+    u[k][2] = ez / n;
+  }
+}
+
+// One step (Euler, or RK4 for Rk4 sources) for two points at once, FP64
+// throughout.
+template <class Src>
+__device__ __forceinline__ void step2_f64(const double (&p)[2][3], double (&o)[2][3],
+                                          const float4* __restrict__ ch, int npad, double h,
+                                          EpiShared& sh, int wave, int lane) {
+  if constexpr (!IsRk4<Src>::value) {
+    double e[2][3];
+    field2_split<Src>(p, e, ch, npad, sh, wave, lane);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const double ex = kCoulomb * e[k][0], ey = kCoulomb * e[k][1],
+                   ez = kCoulomb * e[k][2];
+      const double n = sqrt(ex * ex + ey * ey + ez * ez);
+      o[k][0] = p[k][0] + h * ex / n;
+      o[k][1] = p[k][1] + h * ey / n;
+      o[k][2] = p[k][2] + h * ez / n;
+    }
+  } else {
+    double u[2][3], q[2][3], acc[2][3];
+    unit2_f64<Src>(p, u, ch, npad, sh, wave, lane);
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        acc[k][c] = u[k][c];
+        q[k][c] = p[k][c] + 0.5 * h * u[k][c];
+      }
+    unit2_f64<Src>(q, u, ch, npad, sh, wave, lane);
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        acc[k][c] += 2.0 * u[k][c];
+        q[k][c] = p[k][c] + 0.5 * h * u[k][c];
+      }
+    unit2_f64<Src>(q, u, ch, npad, sh, wave, lane);
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        acc[k][c] += 2.0 * u[k][c];
+        q[k][c] = p[k][c] + h * u[k][c];
+      }
+    unit2_f64<Src>(q, u, ch, npad, sh, wave, lane);
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) o[k][c] = p[k][c] + (h / 6.0) * (acc[k][c] + u[k][c]);
   }
 }
 
@@ -853,6 +978,7 @@ int topo_run(cpet_ctx* ctx, int N, int L, const float* starts,
     return CPET_ERR_ARG;
   }
   hipError_t e = src == kSrcCharge       ? launch_prop_s<ChargeSrc>(S, st, G, ch, pa)
+                 : src == kSrcChargeRk4  ? launch_prop_s<Rk4<ChargeSrc>>(S, st, G, ch, pa)
                  : src == kSrcDipoleRef  ? launch_prop_s<DipoleSrc<false>>(S, st, G, ch, pa)
                                          : launch_prop_s<DipoleSrc<true>>(S, st, G, ch, pa);
   if (e != hipSuccess) {
@@ -883,6 +1009,9 @@ int topo_run(cpet_ctx* ctx, int N, int L, const float* starts,
     hipLaunchKernelGGL(topo_epilogue_kernel<ChargeSrc>, dim3((L + kEpiLanes - 1) / kEpiLanes),
                        dim3(kEpiWaves * kEpiLanes), lds,
                        st, ch, ea);
+  else if (src == kSrcChargeRk4)
+    hipLaunchKernelGGL(topo_epilogue_kernel<Rk4<ChargeSrc>>, dim3((L + kEpiLanes - 1) / kEpiLanes),
+                       dim3(kEpiWaves * kEpiLanes), lds, st, ch, ea);
   else if (src == kSrcDipoleRef)
     hipLaunchKernelGGL(topo_epilogue_kernel<DipoleSrc<false>>, dim3((L + kEpiLanes - 1) / kEpiLanes),
                        dim3(kEpiWaves * kEpiLanes),

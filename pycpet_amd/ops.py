@@ -131,22 +131,26 @@ def field_points(pts, x, q, *, want_E: bool = True, want_phi: bool = False,
 @dataclass
 class TopoResult:
     feats: Any            # (L,2) float64: distance, mean curvature (input order)
-    steps: Any            # (L,) int32 Euler steps taken
+    steps: Any            # (L,) int32 steps taken (Euler or RK4)
     exited: Any           # (L,) int32: 1 = left the box, 0 = hit n_iter
     hist: Any = None      # (nb_d, nb_c) uint32 or None
 
 
-def topo(starts, n_iter, h: float, dims, x, q, *, edges=None,
+def topo(starts, n_iter, h: float, dims, x, q, *, edges=None, integrator: str = "euler",
          ctx: Optional[nat.Context] = None) -> TopoResult:
     """Streamlines from every start (thread_operation semantics).
 
     edges: optional (edges_d, edges_c) increasing float64 bin edges; the
     histogram counts the float32-rounded features with numpy.histogramdd
     rules.  dims is read on the host in every mode (pass a numpy array, or
-    a tensor, which is copied to the host).
+    a tensor, which is copied to the host).  integrator: "euler" (the
+    reference's) or "rk4" (classical RK4 on E/|E| for the walk and the
+    curvature steps, an extension).
     """
     ctx = ctx or nat.default_context()
-    flags = 0
+    if integrator not in ("euler", "rk4"):
+        raise ValueError(f"unknown integrator {integrator!r}")
+    flags = nat.F_TOPO_RK4 if integrator == "rk4" else 0
     h = float(np.float32(h))
     if _is_torch(dims):
         dims = dims.detach().cpu().numpy()

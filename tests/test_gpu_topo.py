@@ -178,3 +178,36 @@ def test_full_size_100k(gpu_ctx, oracle_mod):
     dd = np.abs(sub.feats[m, 0] - f64[m, 0]) / f64[m, 0]
     dk = np.abs(sub.feats[m, 1] - f64[m, 1]) / np.maximum(np.abs(f64[m, 1]), 1e-6)
     assert dd.max() <= FEAT_TOL and dk.max() <= FEAT_TOL
+
+
+@pytest.mark.parametrize("N,L", [(300, 512), (800, 384)])
+def test_rk4_vs_oracle(gpu_ctx, oracle_mod, N, L):
+    """integrator="rk4" against the FP64 RK4 oracle, with the same matching
+    rule as the Euler kernel."""
+    from pycpet_amd import ops, workloads as W
+
+    c = W.small_topo(N=N, L=L)
+    r = ops.topo(c.starts, c.n_iter, c.h, c.dims, c.x, c.q, integrator="rk4", ctx=gpu_ctx)
+    f64, s64, e64 = oracle_mod.topo_rk4_f64(c.starts, c.n_iter, c.h, c.dims, c.x, c.q)
+    compare(r, f64, s64, e64, f"rk4_{N}x{L}")
+
+
+def test_rk4_radial_lines_equal_euler(gpu_ctx):
+    """A single charge: straight radial lines, so RK4 and Euler take the same
+    steps (the RK4 stage bookkeeping must not change the step count)."""
+    from pycpet_amd import ops
+
+    x = np.array([[0.0, 0.0, 0.0]], np.float32)
+    q = np.array([1.0], np.float32)
+    rng = np.random.default_rng(4)
+    d = rng.standard_normal((300, 3))
+    starts = (0.3 * d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    n_iter = rng.integers(0, 300, 300).astype(np.int32)
+    dims = np.full(3, 1.0, np.float32)
+    e = ops.topo(starts, n_iter, 0.01, dims, x, q, ctx=gpu_ctx)
+    k = ops.topo(starts, n_iter, 0.01, dims, x, q, integrator="rk4", ctx=gpu_ctx)
+    np.testing.assert_array_equal(k.steps, e.steps)
+    np.testing.assert_array_equal(k.exited, e.exited)
+    np.testing.assert_allclose(k.feats[:, 0], e.feats[:, 0], rtol=1e-9, atol=1e-12)
+    with pytest.raises(ValueError):
+        ops.topo(starts, n_iter, 0.01, dims, x, q, integrator="rk2", ctx=gpu_ctx)
