@@ -17,6 +17,9 @@ per launch / the kernel's mean duration, against the FP32 vector peak.
 `cpu_baseline` times the reference's own thread_operation (built from its
 source under oracle/_ref) on a bounded seeded sample of the same streamlines
 on the host cores, or the oracle's FP32 port when that build is absent.
+Secondary lines: `volume_30k` (the 101^3 grid over 30,000 clustered
+charges) and `topo_rk4` (the same streamlines integrated with RK4, four
+field evaluations per step; skip with --no-rk4).
 """
 from __future__ import annotations
 
@@ -102,6 +105,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-volume", action="store_true")
+    ap.add_argument("--no-rk4", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
@@ -203,6 +207,29 @@ def main() -> int:
             "recompute_ms": float(np.mean(fms)),
             "recomputed_points": ctx.refined_points(),
             "achieved_tflops": FLOP_PER_PAIR * vpairs / (np.mean(kms) * 1e-3) / 1e12,
+        }
+
+    if not args.no_rk4:
+        # the same streamline workload with the RK4 integrator (an extension:
+        # four field evaluations per step, the curvature steps RK4 as well)
+        ops.topo(starts, n_iter, case.h, dims, x, q, integrator="rk4", ctx=ctx)
+        torch.cuda.synchronize()
+        kms = []
+        t2 = time.perf_counter()
+        for _ in range(2):
+            rk = ops.topo(starts, n_iter, case.h, dims, x, q, integrator="rk4", ctx=ctx)
+            kms.append(ctx.phase_times(3)[1])
+        torch.cuda.synchronize()
+        rwall = (time.perf_counter() - t2) / 2
+        rsteps = int(rk.steps.cpu().numpy().astype(np.int64).sum())
+        rpairs = float(4 * rsteps + 16 * case.L) * case.N
+        out["topo_rk4"] = {
+            "pair_evals_per_s": rpairs / rwall,
+            "ms_per_step": 1e3 * rwall,
+            "streamlines_per_s": case.L / rwall,
+            "rk4_steps": rsteps,
+            "kernel_ms": float(np.mean(kms)),
+            "achieved_tflops": FLOP_PER_PAIR * 4 * rsteps * case.N / (np.mean(kms) * 1e-3) / 1e12,
         }
 
     if not args.no_cpu_baseline and int(os.environ.get("RANK", "0")) == 0:
