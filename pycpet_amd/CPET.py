@@ -14,6 +14,8 @@ option of this build, default 16; 1 gives the reference's one frame per
 call) through one batched streamline launch and then write their .top
 files.  The batch computes each frame exactly as the per-frame call does, so
 the files are the same; frames that alone cannot fill the GPU run together.
+With the option integrator = "rk4" (an extension; the reference integrates
+with Euler) frames run one per call.
 """
 from __future__ import annotations
 
@@ -88,7 +90,7 @@ class CPET:
     def _topo_group(self, group, runtype):
         """Streamline features of the frames of `group` ((frame, calculator)
         pairs), one batched launch for several, then their .top files."""
-        if len(group) > 1:
+        if len(group) > 1 and all(c.integrator == "euler" for _, c in group):
             from .batch import batch_topo
 
             results = batch_topo([{"x": c.x, "q": c.Q, "starts": c.random_start_points,
@@ -97,9 +99,10 @@ class CPET:
             for (_, c), r in zip(group, results):
                 c.topo_result = r
                 c.hist = r.feats.astype(np.float32)
+        batched = len(group) > 1 and all(c.integrator == "euler" for _, c in group)
         for frame, c in group:
             self.calculator = c
-            hist = c.hist if len(group) > 1 else getattr(c, runtype)()
+            hist = c.hist if batched else getattr(c, runtype)()
             write_rows(self.outputpath + f"/{frame}.top", hist, "%.18e")
 
     def _meta(self, mesh_shape):

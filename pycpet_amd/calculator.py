@@ -45,6 +45,7 @@ class calculator:
         self.path_to_pdb = path_to_pdb
         self.frame_backend = options.get("frame_backend", "gpu")
         self.precision = options.get("precision", "auto")
+        self.integrator = options.get("integrator", "euler")  # "rk4": extension
 
         self.step_size = options["step_size"]
         self.dimensions = np.array(options["dimensions"]) if "dimensions" in options.keys() else None
@@ -240,7 +241,8 @@ class calculator:
     def _topo(self):
         t0 = time.time()
         r = ops.topo(self.random_start_points, np.asarray(self.random_max_samples),
-                     self.step_size, self.dimensions, self.x, self.Q)
+                     self.step_size, self.dimensions, self.x, self.Q,
+                     integrator=self.integrator)
         self.topo_result = r
         self.hist = r.feats.astype(np.float32)
         self.log.debug("Time taken for %d streamlines with N_charges = %d: %.3f s",
