@@ -242,7 +242,7 @@ class calculator:
         t0 = time.time()
         r = ops.topo(self.random_start_points, np.asarray(self.random_max_samples),
                      self.step_size, self.dimensions, self.x, self.Q,
-                     integrator=self.integrator)
+                     integrator=getattr(self, "integrator", "euler"))
         self.topo_result = r
         self.hist = r.feats.astype(np.float32)
         self.log.debug("Time taken for %d streamlines with N_charges = %d: %.3f s",
