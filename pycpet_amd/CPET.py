@@ -90,7 +90,9 @@ class CPET:
     def _topo_group(self, group, runtype):
         """Streamline features of the frames of `group` ((frame, calculator)
         pairs), one batched launch for several, then their .top files."""
-        if len(group) > 1 and all(getattr(c, "integrator", "euler") == "euler" for _, c in group):
+        euler = all(getattr(c, "integrator", "euler") == "euler" for _, c in group)
+        batched = len(group) > 1 and euler  # the batch kernel integrates with Euler
+        if batched:
             from .batch import batch_topo
 
             results = batch_topo([{"x": c.x, "q": c.Q, "starts": c.random_start_points,
@@ -99,7 +101,6 @@ class CPET:
             for (_, c), r in zip(group, results):
                 c.topo_result = r
                 c.hist = r.feats.astype(np.float32)
-        batched = len(group) > 1 and all(getattr(c, "integrator", "euler") == "euler" for _, c in group)
         for frame, c in group:
             self.calculator = c
             hist = c.hist if batched else getattr(c, runtype)()
