@@ -200,3 +200,27 @@ def test_topo_frames_batched_write_the_same_files(tmp_path, gpu_ctx):
         CPET(opts).run()
         texts[per] = {p.name: p.read_bytes() for p in sorted(out.glob("*.top"))}
     assert len(texts[1]) == 3 and texts[1] == texts[3]
+
+
+def test_topo_driver_rk4_option(tmp_path, gpu_ctx):
+    """The integrator option reaches the kernel: the driver's .top rows equal
+    the RK4 features of the same calculator, and differ from Euler's."""
+    from pycpet_amd.CPET import CPET
+    from pycpet_amd.calculator import calculator
+    from pycpet_amd.io import write_rows
+
+    (tmp_path / "in").mkdir()
+    for i in range(2):
+        synthetic_frame(tmp_path / "in" / f"frame{i}.pdb", n=400, seed=20 + i)
+    opts = _options(tmp_path, "topo", n_samples=64, step_size=0.01,
+                    max_streamline_init="fixed_rand", dimensions=[0.5, 0.5, 0.5],
+                    integrator="rk4")
+    CPET(opts).run()
+    for i in range(2):
+        f = str(tmp_path / "in" / f"frame{i}.pdb")
+        rk = calculator(opts, f).compute_topo_complete_c_shared()
+        eu = calculator(dict(opts, integrator="euler"), f).compute_topo_complete_c_shared()
+        want = tmp_path / f"want{i}.top"
+        write_rows(str(want), rk, "%.18e")
+        assert (tmp_path / "out" / f"frame{i}.top").read_bytes() == want.read_bytes()
+        assert not np.array_equal(rk, eu)
