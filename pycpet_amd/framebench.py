@@ -2,6 +2,7 @@
 
     python -m pycpet_amd.framebench batch --frames 64
     python -m pycpet_amd.framebench serve --rate 200 --duration 10
+    python -m pycpet_amd.framebench trajectory --frames 2000
 
 batch: the mixed MD frame set (pycpet_amd.workloads.md_frames: half volume
 grids of edge 21/41/61/81, half 20,000-start streamline jobs, 3k-40k atoms
@@ -14,6 +15,8 @@ per frame), inputs resident in HBM, timed three ways:
             persistent launch, on two streams;
   batched_graph  the same two batched launches captured as parallel
             branches of one HIP graph (pycpet_amd.graphs) and replayed.
+trajectory: many small frames of one shape (2,000 charges, 21^3 grid), each
+computed eagerly or by replaying one captured HIP graph.
 serve: Poisson arrivals (pycpet_amd.serve.poisson_requests) served with
 continuous batching vs one request at a time (stopped after --plain-max-s,
 unserved requests counted with their latency lower bound); p50/p99 latency
@@ -151,6 +154,57 @@ def run_batch(n_frames: int, topo_starts: int, seed: int = 3) -> dict:
     return out
 
 
+def run_trajectory(n_frames: int, seed: int = 0) -> dict:
+    """A trajectory of small frames of one shape (2,000 charges, 21^3 grid,
+    field and potential): each frame's coordinates are copied into the same
+    device tensors and the frame is computed eagerly (ops.field_grid) or by
+    replaying one captured HIP graph (pycpet_amd.graphs.CapturedStep).  At
+    this size the launches, not the arithmetic, set the frame time."""
+    import torch
+
+    from . import _native as nat, graphs as G, ops, workloads as W
+
+    c = W.volume_2k()
+    dev = torch.device("cuda:0")
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    ax = [T(a) for a in c.axes]
+    x, q = T(c.x), T(c.q)
+    rng = np.random.default_rng(seed)
+    traj = T(np.stack([c.x + rng.normal(0, 0.05, c.x.shape).astype(np.float32)
+                       for _ in range(min(n_frames, 64))]))
+    ctx = nat.Context(0)
+    ctx.set_profiling(False)
+    step = G.CapturedStep(lambda cx: ops.field_grid(*ax, x, q, want_phi=True, ctx=cx))
+    out = {"workload": "trajectory_2k_charges_21cubed", "frames": n_frames}
+    for name in ("eager", "graph"):
+        for _ in range(3):  # warm-up
+            if name == "eager":
+                ops.field_grid(*ax, x, q, want_phi=True, ctx=ctx)
+            else:
+                step.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(n_frames):
+            x.copy_(traj[i % traj.shape[0]])
+            if name == "eager":
+                ops.field_grid(*ax, x, q, want_phi=True, ctx=ctx)
+            else:
+                step.replay()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out[f"{name}_us_per_frame"] = 1e6 * dt / n_frames
+        out[f"{name}_frames_per_s"] = n_frames / dt
+        print(f"{name}: {1e6 * dt / n_frames:.1f} us per frame", file=sys.stderr, flush=True)
+    # the last replay equals an eager run on the same coordinates
+    E, phi = step.replay()
+    Ee, pe = ops.field_grid(*ax, x, q, want_phi=True, ctx=ctx)
+    torch.cuda.synchronize()
+    out["graph_equals_eager"] = bool(torch.equal(E, Ee) and torch.equal(phi, pe))
+    out["graph_speedup"] = out["eager_us_per_frame"] / out["graph_us_per_frame"]
+    step.close()
+    return out
+
+
 def run_serve(rate: float, duration: float, lanes: int, seed: int = 0,
               plain_max_s: float = 120.0, policy: str = "fifo-small") -> dict:
     import torch
@@ -193,9 +247,13 @@ def main(argv=None) -> int:
     s.add_argument("--policy", default="fifo-small", choices=("fifo-small", "greedy"))
     s.add_argument("--plain-max-s", type=float, default=120.0,
                    help="stop the one-at-a-time baseline after this wall time")
+    t = sub.add_parser("trajectory")
+    t.add_argument("--frames", type=int, default=2000)
     a = ap.parse_args(argv)
     if a.cmd == "batch":
         out = run_batch(a.frames, a.topo_starts)
+    elif a.cmd == "trajectory":
+        out = run_trajectory(a.frames)
     else:
         out = run_serve(a.rate, a.duration, a.lanes, plain_max_s=a.plain_max_s, policy=a.policy)
     print(json.dumps(out), flush=True)
