@@ -162,7 +162,10 @@ int cpet_field_grid(cpet_ctx *ctx, int nx, int ny, int nz, const float *ax,
  *          given edges (nb_d+1 and nb_c+1 doubles, increasing), as
  *          np.histogram2d does in CPET/utils/calculator.py:702-707.
  * Positions are integrated in FP64 with FP32 tiled field sums; the four
- * curvature-defining evaluations run in FP64.
+ * curvature-defining evaluations run in FP64.  With CPET_F_PREC_F64 the
+ * field sums of the walk are FP64 as well (a precision reference, slower);
+ * the other precision modes give the FP32 tiled sums.  CPET_F_TOPO_RK4
+ * selects RK4 steps (see the flag).
  * dims (like h) is a host array in every mode; with CPET_F_DEVICE the other
  * arrays, the edges included, are device pointers, and with CPET_F_NO_SYNC
  * as well the call neither waits on nor reads back from the stream (so it

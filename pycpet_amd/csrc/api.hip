@@ -616,7 +616,9 @@ int cpet_topo(cpet_ctx* ctx, int N, int L, const float* starts,
   if (rc) return rc;
   return topo_run(ctx, N, L, starts, n_iter, h, hdims, ch, npad, feats, steps,
                   exited, nb_d, edges_d, nb_c, edges_c, hist, flags,
-                  (flags & CPET_F_TOPO_RK4) ? kSrcChargeRk4 : kSrcCharge);
+                  (flags & CPET_F_PREC_MASK) == CPET_F_PREC_F64
+                      ? ((flags & CPET_F_TOPO_RK4) ? kSrcChargeF64Rk4 : kSrcChargeF64)
+                      : ((flags & CPET_F_TOPO_RK4) ? kSrcChargeRk4 : kSrcCharge));
 }
 
 int cpet_topo_dipole(cpet_ctx* ctx, int N, int L, const float* starts, const int32_t* n_iter,

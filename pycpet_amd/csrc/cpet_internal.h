@@ -36,7 +36,14 @@ enum Mode : int {
 // What a field launch computes.
 // Streamline field sources (topo_run): charges, or point dipoles with the
 // reference's or the physical pair term.
-enum SrcKind : int { kSrcCharge = 0, kSrcDipoleRef = 1, kSrcDipolePhys = 2, kSrcChargeRk4 = 3 };
+enum SrcKind : int {
+  kSrcCharge = 0,
+  kSrcDipoleRef = 1,
+  kSrcDipolePhys = 2,
+  kSrcChargeRk4 = 3,
+  kSrcChargeF64 = 4,     // propagate sum in FP64 (CPET_F_PREC_F64)
+  kSrcChargeF64Rk4 = 5
+};
 
 enum What : int {
   kWhatESoft = 0,     // E with the r^2 >= 1e-6 floor (volume)

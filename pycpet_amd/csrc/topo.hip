@@ -390,6 +390,44 @@ struct IsRk4<Rk4<Base>> {
   static constexpr bool value = true;
 };
 
+// Charges with the propagate sum in FP64 throughout (CPET_F_PREC_F64): the
+// float32 charges are converted exactly, pair terms and sums are FP64, and
+// 1/r comes from v_rsq_f64 with two Newton steps.  A precision reference on
+// the GPU, not a fast path.  The epilogue is the charge policy's (FP64).
+struct ChargeSrcF64 : ChargeSrc {
+  __device__ static void slice64(double px, double py, double pz, const float4* __restrict__ ch,
+                                 int t0, int t1, double& ex, double& ey, double& ez) {
+    constexpr int kGroup = 4;
+    ex = ey = ez = 0.0;
+    for (int j = t0 * kTile; j < t1 * kTile; j += kGroup) {
+      float4 c[kGroup];
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u) c[u] = ch[j + u];
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u) {
+        const double dx = px - (double)c[u].x, dy = py - (double)c[u].y,
+                     dz = pz - (double)c[u].z;
+        const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        const double ri = rsqrt_f64(r2);
+        const double s = ((double)c[u].w * ri) * (ri * ri);
+        ex = fma(s, dx, ex);
+        ey = fma(s, dy, ey);
+        ez = fma(s, dz, ez);
+      }
+    }
+  }
+};
+template <class Src>
+struct IsF64 {
+  static constexpr bool value = false;
+};
+template <>
+struct IsF64<ChargeSrcF64> {
+  static constexpr bool value = true;
+};
+template <class Base>
+struct IsF64<Rk4<Base>> : IsF64<Base> {};
+
 enum SlotState : int { kActive = 0, kNeed = 1, kDone = 2 };
 
 // One frame's streamline set as the slot loop sees it.  Streamline indices
@@ -463,8 +501,11 @@ __device__ __forceinline__ void run_slots(const float4* __restrict__ ch, const F
   while (__ballot(st != kDone) != 0ull) {
     double fx = 0.0, fy = 0.0, fz = 0.0;
     if (st == kActive)
-      Src::slice((float)(rk4 ? qx : px), (float)(rk4 ? qy : py), (float)(rk4 ? qz : pz), ch,
-                 f.t0, f.t1, fx, fy, fz);
+      if constexpr (IsF64<Src>::value)
+        Src::slice64(rk4 ? qx : px, rk4 ? qy : py, rk4 ? qz : pz, ch, f.t0, f.t1, fx, fy, fz);
+      else
+        Src::slice((float)(rk4 ? qx : px), (float)(rk4 ? qy : py), (float)(rk4 ? qz : pz), ch,
+                   f.t0, f.t1, fx, fy, fz);
     part[par][wave][0][lane] = fx;
     part[par][wave][1][lane] = fy;
     part[par][wave][2][lane] = fz;
@@ -979,6 +1020,8 @@ int topo_run(cpet_ctx* ctx, int N, int L, const float* starts,
   }
   hipError_t e = src == kSrcCharge       ? launch_prop_s<ChargeSrc>(S, st, G, ch, pa)
                  : src == kSrcChargeRk4  ? launch_prop_s<Rk4<ChargeSrc>>(S, st, G, ch, pa)
+                 : src == kSrcChargeF64  ? launch_prop_s<ChargeSrcF64>(S, st, G, ch, pa)
+                 : src == kSrcChargeF64Rk4 ? launch_prop_s<Rk4<ChargeSrcF64>>(S, st, G, ch, pa)
                  : src == kSrcDipoleRef  ? launch_prop_s<DipoleSrc<false>>(S, st, G, ch, pa)
                                          : launch_prop_s<DipoleSrc<true>>(S, st, G, ch, pa);
   if (e != hipSuccess) {
@@ -1005,11 +1048,12 @@ int topo_run(cpet_ctx* ctx, int N, int L, const float* starts,
     hipLaunchKernelGGL(fill_u32_kernel, dim3((nbins + 255) / 256), dim3(256), 0, st,
                        d_hist, (long long)nbins, 0u);
   const size_t lds = ea.lds_hist ? (size_t)nbins * sizeof(unsigned int) : 0;
-  if (src == kSrcCharge)
+  // the FP64-sum variants share the charge epilogues (already FP64)
+  if (src == kSrcCharge || src == kSrcChargeF64)
     hipLaunchKernelGGL(topo_epilogue_kernel<ChargeSrc>, dim3((L + kEpiLanes - 1) / kEpiLanes),
                        dim3(kEpiWaves * kEpiLanes), lds,
                        st, ch, ea);
-  else if (src == kSrcChargeRk4)
+  else if (src == kSrcChargeRk4 || src == kSrcChargeF64Rk4)
     hipLaunchKernelGGL(topo_epilogue_kernel<Rk4<ChargeSrc>>, dim3((L + kEpiLanes - 1) / kEpiLanes),
                        dim3(kEpiWaves * kEpiLanes), lds, st, ch, ea);
   else if (src == kSrcDipoleRef)

@@ -137,7 +137,7 @@ class TopoResult:
 
 
 def topo(starts, n_iter, h: float, dims, x, q, *, edges=None, integrator: str = "euler",
-         ctx: Optional[nat.Context] = None) -> TopoResult:
+         prec: str = "auto", ctx: Optional[nat.Context] = None) -> TopoResult:
     """Streamlines from every start (thread_operation semantics).
 
     edges: optional (edges_d, edges_c) increasing float64 bin edges; the
@@ -145,12 +145,17 @@ def topo(starts, n_iter, h: float, dims, x, q, *, edges=None, integrator: str = 
     rules.  dims is read on the host in every mode (pass a numpy array, or
     a tensor, which is copied to the host).  integrator: "euler" (the
     reference's) or "rk4" (classical RK4 on E/|E| for the walk and the
-    curvature steps, an extension).
+    curvature steps, an extension).  prec: "auto" (FP32 tiled field sums,
+    FP64 positions) or "f64" (FP64 field sums, a precision reference).
     """
     ctx = ctx or nat.default_context()
     if integrator not in ("euler", "rk4"):
         raise ValueError(f"unknown integrator {integrator!r}")
+    if prec not in ("auto", "f64"):
+        raise ValueError(f"streamline precision must be 'auto' or 'f64' (got {prec!r})")
     flags = nat.F_TOPO_RK4 if integrator == "rk4" else 0
+    if prec == "f64":
+        flags |= nat.PREC["f64"]
     h = float(np.float32(h))
     if _is_torch(dims):
         dims = dims.detach().cpu().numpy()

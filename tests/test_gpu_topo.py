@@ -211,3 +211,21 @@ def test_rk4_radial_lines_equal_euler(gpu_ctx):
     np.testing.assert_allclose(k.feats[:, 0], e.feats[:, 0], rtol=1e-9, atol=1e-12)
     with pytest.raises(ValueError):
         ops.topo(starts, n_iter, 0.01, dims, x, q, integrator="rk2", ctx=gpu_ctx)
+
+
+def test_f64_mode_vs_oracle(gpu_ctx, oracle_mod):
+    """prec="f64": FP64 field sums in the walk.  Against the FP64 oracle the
+    features agree far inside the FP32 path's errors (Euler and RK4)."""
+    from pycpet_amd import ops, workloads as W
+
+    c = W.small_topo(N=500, L=256)
+    for integ, ref in (("euler", oracle_mod.topo_f64), ("rk4", oracle_mod.topo_rk4_f64)):
+        f64, s64, e64 = ref(c.starts, c.n_iter, c.h, c.dims, c.x, c.q)
+        r = ops.topo(c.starts, c.n_iter, c.h, c.dims, c.x, c.q, integrator=integ, prec="f64",
+                     ctx=gpu_ctx)
+        m = compare(r, f64, s64, e64, f"f64_{integ}")
+        assert m.all()  # FP64 sums: every step count and stop kind equal
+        dd = np.abs(r.feats[:, 0] - f64[:, 0]) / np.maximum(np.abs(f64[:, 0]), 1e-12)
+        assert dd.max() <= 1e-10, dd.max()
+    with pytest.raises(ValueError):
+        ops.topo(c.starts, c.n_iter, c.h, c.dims, c.x, c.q, prec="kahan", ctx=gpu_ctx)
