@@ -18,8 +18,10 @@ per launch / the kernel's mean duration, against the FP32 vector peak.
 source under oracle/_ref) on a bounded seeded sample of the same streamlines
 on the host cores, or the oracle's FP32 port when that build is absent.
 Secondary lines: `volume_30k` (the 101^3 grid over 30,000 clustered
-charges) and `topo_rk4` (the same streamlines integrated with RK4, four
-field evaluations per step; skip with --no-rk4).
+charges), `topo_f64` (the same streamlines with FP64 field sums, and the
+FP32 path's match fraction and feature errors against it; --no-f64) and
+`topo_rk4` (the same streamlines integrated with RK4, four field
+evaluations per step; --no-rk4).
 """
 from __future__ import annotations
 
@@ -106,6 +108,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-volume", action="store_true")
     ap.add_argument("--no-rk4", action="store_true")
+    ap.add_argument("--no-f64", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
@@ -207,6 +210,33 @@ def main() -> int:
             "recompute_ms": float(np.mean(fms)),
             "recomputed_points": ctx.refined_points(),
             "achieved_tflops": FLOP_PER_PAIR * vpairs / (np.mean(kms) * 1e-3) / 1e12,
+        }
+
+    if not args.no_f64:
+        # the same streamlines with FP64 field sums: its speed, and the FP32
+        # path's error against it over all 100,000 streamlines (the FP64 mode
+        # is within 4e-15 of the CPU FP64 oracle on the parity tests)
+        ops.topo(starts, n_iter, case.h, dims, x, q, prec="f64", ctx=ctx)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        r64 = ops.topo(starts, n_iter, case.h, dims, x, q, prec="f64", ctx=ctx)
+        k64 = ctx.phase_times(3)[1]
+        torch.cuda.synchronize()
+        w64 = time.perf_counter() - t3
+        s64 = r64.steps.cpu().numpy()
+        f64 = r64.feats.cpu().numpy()
+        f32 = r.feats.cpu().numpy()
+        m = (steps_np == s64) & (r.exited.cpu().numpy() == r64.exited.cpu().numpy())
+        dd = np.abs(f32[m, 0] - f64[m, 0]) / np.maximum(np.abs(f64[m, 0]), 1e-12)
+        dk = np.abs(f32[m, 1] - f64[m, 1]) / np.maximum(np.abs(f64[m, 1]), 1e-6)
+        out["topo_f64"] = {
+            "pair_evals_per_s": float(int(s64.astype(np.int64).sum()) + 4 * case.L) * case.N / w64,
+            "kernel_ms": k64,
+            "f32_vs_f64": {"match_fraction": float(m.mean()),
+                           "distance_rel_max": float(dd.max()),
+                           "distance_rel_rms": float(np.sqrt(np.mean(dd ** 2))),
+                           "curvature_rel_max": float(dk.max()),
+                           "curvature_rel_rms": float(np.sqrt(np.mean(dk ** 2)))},
         }
 
     if not args.no_rk4:
