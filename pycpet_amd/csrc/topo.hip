@@ -448,13 +448,17 @@ struct FrameView {
 // Runs the workgroup's 64 slots until the frame's queue is exhausted.  Slot
 // state is identical in every wave: each wave sums its charge tiles for all
 // 64 slots, the partials are reduced in a fixed order through LDS, and all
-// waves apply the same FP64 update.  A finished slot asks the queue for the
+// waves apply the same FP64 update.  With S >= 4 waves, waves 0-2 reduce one
+// field component each into esum and the others wait at a second barrier:
+// one pass over the partials instead of one per wave, which would be S times
+// the LDS reads of the same sums.  A finished slot asks the queue for the
 // next streamline (one idle iteration).  first_q >= 0 gives the slot an
 // initial queue position, first_q < 0 starts it empty.
 template <int S, class Src>
 __device__ __forceinline__ void run_slots(const float4* __restrict__ ch, const FrameView& f,
                                           int first_q, double (*part)[S][3][64],
-                                          int (*nsid)[64], int lane, int wave) {
+                                          double (*esum)[64], int (*nsid)[64], int lane,
+                                          int wave) {
   constexpr bool rk4 = IsRk4<Src>::value;
   int st = kDone, sid = 0, step = 0, nmax = 0;
   double px = 0.0, py = 0.0, pz = 0.0;
@@ -512,13 +516,30 @@ __device__ __forceinline__ void run_slots(const float4* __restrict__ ch, const F
     if (wave == 0 && st == kNeed)
       nsid[par][lane] = (int)atomicAdd(f.counter, 1u);
     __syncthreads();
+    if constexpr (S >= 4) {
+      // esum is read after the second barrier and rewritten only after the
+      // next step's first barrier, which every reader has passed by then.
+      if (wave < 3) {
+        double e = 0.0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) e += part[par][s][wave][lane];
+        esum[wave][lane] = e;
+      }
+      __syncthreads();
+    }
     if (st == kActive) {
       double ex = 0.0, ey = 0.0, ez = 0.0;
+      if constexpr (S >= 4) {
+        ex = esum[0][lane];
+        ey = esum[1][lane];
+        ez = esum[2][lane];
+      } else {
 #pragma unroll
-      for (int s = 0; s < S; ++s) {
-        ex += part[par][s][0][lane];
-        ey += part[par][s][1][lane];
-        ez += part[par][s][2][lane];
+        for (int s = 0; s < S; ++s) {
+          ex += part[par][s][0][lane];
+          ey += part[par][s][1][lane];
+          ez += part[par][s][2][lane];
+        }
       }
       ex *= kCoulomb;
       ey *= kCoulomb;
@@ -584,6 +605,7 @@ template <int S, class Src>
 __global__ __launch_bounds__(64 * S, 8) void topo_propagate_kernel(
     const float4* __restrict__ ch, PropArgs a) {
   __shared__ double part[2][S][3][64];
+  __shared__ double esum[3][64];
   __shared__ int nsid[2][64];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -604,7 +626,7 @@ __global__ __launch_bounds__(64 * S, 8) void topo_propagate_kernel(
   f.counter = a.counter;
   f.t0 = min(wave * tpw, ntiles);
   f.t1 = min(f.t0 + tpw, ntiles);
-  run_slots<S, Src>(ch, f, blockIdx.x * 64 + lane, part, nsid, lane, wave);
+  run_slots<S, Src>(ch, f, blockIdx.x * 64 + lane, part, esum, nsid, lane, wave);
 }
 
 // Many frames in one persistent launch.  Workgroups claim assignments
@@ -634,6 +656,7 @@ template <int S>
 __global__ __launch_bounds__(64 * S, 8) void topo_batch_kernel(const float4* __restrict__ ch,
                                                             BatchTopoArgs a) {
   __shared__ double part[2][S][3][64];
+  __shared__ double esum[3][64];
   __shared__ int nsid[2][64];
   __shared__ int cur;
   const int lane = threadIdx.x & 63;
@@ -664,7 +687,7 @@ __global__ __launch_bounds__(64 * S, 8) void topo_batch_kernel(const float4* __r
     f.counter = a.sl_counter + fr;
     f.t0 = c0 + min(wave * tpw, ntiles);
     f.t1 = c0 + min(wave * tpw + tpw, ntiles);
-    run_slots<S, ChargeSrc>(ch, f, -1, part, nsid, lane, wave);
+    run_slots<S, ChargeSrc>(ch, f, -1, part, esum, nsid, lane, wave);
   }
 }
 
