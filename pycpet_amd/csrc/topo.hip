@@ -504,12 +504,13 @@ __device__ __forceinline__ void run_slots(const float4* __restrict__ ch, const F
   int par = 0;
   while (__ballot(st != kDone) != 0ull) {
     double fx = 0.0, fy = 0.0, fz = 0.0;
-    if (st == kActive)
+    if (st == kActive) {
       if constexpr (IsF64<Src>::value)
         Src::slice64(rk4 ? qx : px, rk4 ? qy : py, rk4 ? qz : pz, ch, f.t0, f.t1, fx, fy, fz);
       else
         Src::slice((float)(rk4 ? qx : px), (float)(rk4 ? qy : py), (float)(rk4 ? qz : pz), ch,
                    f.t0, f.t1, fx, fy, fz);
+    }
     part[par][wave][0][lane] = fx;
     part[par][wave][1][lane] = fy;
     part[par][wave][2][lane] = fz;
