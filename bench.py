@@ -18,7 +18,8 @@ per launch / the kernel's mean duration, against the FP32 vector peak.
 source under oracle/_ref) on a bounded seeded sample of the same streamlines
 on the host cores, or the oracle's FP32 port when that build is absent.
 Secondary lines: `volume_30k` (the 101^3 grid over 30,000 clustered
-charges), `topo_f64` (the same streamlines with FP64 field sums, and the
+charges) and `volume_esp_30k` (the same grid, potential only; both
+--no-volume), `topo_f64` (the same streamlines with FP64 field sums, and the
 FP32 path's match fraction and feature errors against it; --no-f64) and
 `topo_rk4` (the same streamlines integrated with RK4, four field
 evaluations per step; --no-rk4).
@@ -210,6 +211,25 @@ def main() -> int:
             "recompute_ms": float(np.mean(fms)),
             "recomputed_points": ctx.refined_points(),
             "achieved_tflops": FLOP_PER_PAIR * vpairs / (np.mean(kms) * 1e-3) / 1e12,
+        }
+        # the same grid, potential only (volume_ESP): mixed FP32/FP64 pair
+        # term, 11 algorithmic FLOP per pair
+        for _ in range(2):
+            ops.field_grid(*ax, vx, vq, want_E=False, want_phi=True, ctx=ctx)
+        torch.cuda.synchronize()
+        kms = []
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            ops.field_grid(*ax, vx, vq, want_E=False, want_phi=True, ctx=ctx)
+            ph = ctx.phase_times(3)
+            kms.append(ph[0] + ph[1])
+        torch.cuda.synchronize()
+        pwall = (time.perf_counter() - t1) / args.steps
+        out["volume_esp_30k"] = {
+            "pair_evals_per_s": vpairs / pwall,
+            "ms_per_frame": 1e3 * pwall,
+            "kernel_ms": float(np.mean(kms)),
+            "achieved_tflops": 11 * vpairs / (np.mean(kms) * 1e-3) / 1e12,
         }
 
     if not args.no_f64:
