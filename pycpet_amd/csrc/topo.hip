@@ -18,7 +18,8 @@
 //      streamlines.
 //   2. propagate (persistent): a workgroup of S waves owns 64 streamline
 //      slots; wave s sums the field of charge slice s for all 64 slots (FP32
-//      per 64-charge tile, FP64 across tiles), the S partials are reduced in
+//      per chunk of up to 64 charges, FP64 across chunks; the slices differ
+//      by at most four charges), the S partials are reduced in
 //      a fixed order through LDS and every wave applies the identical FP64
 //      position update.  A finished slot takes the next id from a global
 //      queue.  Results do not depend on which slot runs a streamline.
@@ -219,48 +220,66 @@ struct PropArgs {
   unsigned int* counter;
 };
 
-// Streams the records of source tiles [t0, t1) (kRec float4 per tile) to
-// body(c, fx, fy, fz), four float4 at a time, summing FP32 per tile and FP64
-// across tiles.  The records pass through two SGPR buffers of four float4
-// (one 64-byte scalar load each).  The load of the next group is issued
-// only after the current group has arrived (an explicit lgkmcnt(0) wait:
-// scalar loads return out of order, so any later wait covers every load in
-// flight), and scheduling barriers keep it ahead of the current group's
-// arithmetic, which then hides its latency.  The stream runs on across tile
-// boundaries; the last prefetch is clamped to the wave's last group, so no
-// load leaves the wave's range.
+// Streams source records [r0, r1) (float4; r0 and r1 multiples of four) to
+// body(c, fx, fy, fz), four records at a time, summing FP32 over chunks of
+// kRec records from r0 and FP64 across chunks.  The records pass through two
+// SGPR buffers of four float4 (one 64-byte scalar load each).  The load of
+// the next group is issued only after the current group has arrived (an
+// explicit lgkmcnt(0) wait: scalar loads return out of order, so any later
+// wait covers every load in flight), and scheduling barriers keep it ahead
+// of the current group's arithmetic, which then hides its latency.  The
+// stream runs on across chunk boundaries; the last prefetch is clamped to
+// the range's last group, so no load leaves the range.  A last partial chunk
+// runs whole group pairs and then at most one single group.
 template <int kRec, class Body>
-__device__ __forceinline__ void stream_tiles(const float4* __restrict__ ch, int t0, int t1,
-                                             Body body, double& ex, double& ey, double& ez) {
+__device__ __forceinline__ void stream_recs(const float4* __restrict__ ch, int r0, int r1,
+                                            Body body, double& ex, double& ey, double& ez) {
   constexpr int kGroup = 4;
   constexpr int kWaitLgkm0 = 0xc07f;  // s_waitcnt lgkmcnt(0), other counters untouched
-  static_assert(kRec % (2 * kGroup) == 0, "a tile holds whole group pairs");
+  static_assert(kRec % (2 * kGroup) == 0, "a chunk holds whole group pairs");
   ex = ey = ez = 0.0;
-  if (t0 >= t1) return;
-  const int last = t1 * kRec - kGroup;  // index of the wave's last group
-  int nx = t0 * kRec;
+  if (r0 >= r1) return;
+  const int last = r1 - kGroup;  // index of the range's last group
+  int nx = r0;
   float4 a[kGroup], b[kGroup];
 #pragma unroll
   for (int u = 0; u < kGroup; ++u) a[u] = ch[nx + u];
-  for (int t = t0; t < t1; ++t) {
+  // One group pair: a has been issued; b (the next group, which exists) is
+  // loaded during a's arithmetic and the group after it into a during b's.
+  auto pair = [&](float& fx, float& fy, float& fz) __attribute__((always_inline)) {
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+    nx += kGroup;
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) b[u] = ch[nx + u];
+    __builtin_amdgcn_sched_barrier(0);
+    body(a, fx, fy, fz);
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+    nx += kGroup;
+    const int pf = min(nx, last);
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) a[u] = ch[pf + u];
+    __builtin_amdgcn_sched_barrier(0);
+    body(b, fx, fy, fz);
+  };
+  int r = r0;
+  for (; r + kRec <= r1; r += kRec) {
     float fx = 0.f, fy = 0.f, fz = 0.f;
 #pragma unroll
-    for (int g = 0; g < kRec; g += 2 * kGroup) {
+    for (int g = 0; g < kRec; g += 2 * kGroup) pair(fx, fy, fz);
+    ex += (double)fx;
+    ey += (double)fy;
+    ez += (double)fz;
+  }
+  if (r < r1) {
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    for (; r + 2 * kGroup <= r1; r += 2 * kGroup) pair(fx, fy, fz);
+    if (r < r1) {  // one group left, already loaded into a
       __builtin_amdgcn_sched_barrier(0);
       __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
-      nx += kGroup;
-#pragma unroll
-      for (int u = 0; u < kGroup; ++u) b[u] = ch[nx + u];
       __builtin_amdgcn_sched_barrier(0);
       body(a, fx, fy, fz);
-      __builtin_amdgcn_sched_barrier(0);
-      __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
-      nx += kGroup;
-      const int pf = min(nx, last);
-#pragma unroll
-      for (int u = 0; u < kGroup; ++u) a[u] = ch[pf + u];
-      __builtin_amdgcn_sched_barrier(0);
-      body(b, fx, fy, fz);
     }
     ex += (double)fx;
     ey += (double)fy;
@@ -268,16 +287,27 @@ __device__ __forceinline__ void stream_tiles(const float4* __restrict__ ch, int 
   }
 }
 
+// Splits n4 groups of four records (from record rec0) evenly over S waves:
+// wave w gets groups [w n4 / S, (w+1) n4 / S), so slices differ by at most
+// one group whatever the source count.
+__device__ __forceinline__ void wave_records(int rec0, int nrec, int wave, int S, int& r0,
+                                             int& r1) {
+  const long long n4 = nrec / 4;
+  r0 = rec0 + (int)(wave * n4 / S) * 4;
+  r1 = rec0 + (int)((wave + 1) * n4 / S) * 4;
+}
+
 // Sources of the field a streamline follows.  A policy gives the FP32
 // slice sum of the propagate kernel and the FP64 field of the epilogue.
 //
 // Charges: one float4 (x, y, z, q) per source (math_module.c:296-333).
 struct ChargeSrc {
-  // Unsoftened field of charge tiles [t0, t1) at one point (the k factor is
-  // applied by the caller).
+  static constexpr int kRecPerSrc = 1;
+  // Unsoftened field of charge records [r0, r1) at one point (the k factor
+  // is applied by the caller).
   __device__ static void slice(float px, float py, float pz, const float4* __restrict__ ch,
-                               int t0, int t1, double& ex, double& ey, double& ez) {
-    stream_tiles<kTile>(ch, t0, t1, [&](const float4 (&c)[4], float& fx, float& fy, float& fz) {
+                               int r0, int r1, double& ex, double& ey, double& ez) {
+    stream_recs<kTile>(ch, r0, r1, [&](const float4 (&c)[4], float& fx, float& fy, float& fz) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const float dx = px - c[u].x, dy = py - c[u].y, dz = pz - c[u].z;
@@ -324,12 +354,13 @@ struct ChargeSrc {
 // PHYS the physical c = 3 mu . R.  Padding dipoles (mu = 0 at 1e18) add 0.
 template <bool PHYS>
 struct DipoleSrc {
+  static constexpr int kRecPerSrc = 2;
   __device__ static float coef(float mx, float my, float mz, float dx, float dy, float dz) {
     return PHYS ? 3.f * fmaf(mz, dz, fmaf(my, dy, mx * dx)) : fmaf(3.f * mx, dx, fmaf(my, dy, mz * dz));
   }
   __device__ static void slice(float px, float py, float pz, const float4* __restrict__ ch,
-                               int t0, int t1, double& ex, double& ey, double& ez) {
-    stream_tiles<2 * kTile>(ch, t0, t1, [&](const float4 (&c)[4], float& fx, float& fy, float& fz) {
+                               int r0, int r1, double& ex, double& ey, double& ez) {
+    stream_recs<2 * kTile>(ch, r0, r1, [&](const float4 (&c)[4], float& fx, float& fy, float& fz) {
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
         const float4 X = c[2 * u], M = c[2 * u + 1];
@@ -396,10 +427,10 @@ struct IsRk4<Rk4<Base>> {
 // the GPU, not a fast path.  The epilogue is the charge policy's (FP64).
 struct ChargeSrcF64 : ChargeSrc {
   __device__ static void slice64(double px, double py, double pz, const float4* __restrict__ ch,
-                                 int t0, int t1, double& ex, double& ey, double& ez) {
+                                 int r0, int r1, double& ex, double& ey, double& ez) {
     constexpr int kGroup = 4;
     ex = ey = ez = 0.0;
-    for (int j = t0 * kTile; j < t1 * kTile; j += kGroup) {
+    for (int j = r0; j < r1; j += kGroup) {
       float4 c[kGroup];
 #pragma unroll
       for (int u = 0; u < kGroup; ++u) c[u] = ch[j + u];
@@ -442,7 +473,7 @@ struct FrameView {
   int32_t* steps;          // (L)
   int32_t* exited;         // (L)
   unsigned int* counter;   // next queue position
-  int t0, t1;              // this wave's charge tiles
+  int r0, r1;              // this wave's source records (float4)
 };
 
 // Runs the workgroup's 64 slots until the frame's queue is exhausted.  Slot
@@ -506,10 +537,10 @@ __device__ __forceinline__ void run_slots(const float4* __restrict__ ch, const F
     double fx = 0.0, fy = 0.0, fz = 0.0;
     if (st == kActive) {
       if constexpr (IsF64<Src>::value)
-        Src::slice64(rk4 ? qx : px, rk4 ? qy : py, rk4 ? qz : pz, ch, f.t0, f.t1, fx, fy, fz);
+        Src::slice64(rk4 ? qx : px, rk4 ? qy : py, rk4 ? qz : pz, ch, f.r0, f.r1, fx, fy, fz);
       else
         Src::slice((float)(rk4 ? qx : px), (float)(rk4 ? qy : py), (float)(rk4 ? qz : pz), ch,
-                   f.t0, f.t1, fx, fy, fz);
+                   f.r0, f.r1, fx, fy, fz);
     }
     part[par][wave][0][lane] = fx;
     part[par][wave][1][lane] = fy;
@@ -610,8 +641,6 @@ __global__ __launch_bounds__(64 * S, 8) void topo_propagate_kernel(
   __shared__ int nsid[2][64];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int ntiles = a.npad / kTile;
-  const int tpw = (ntiles + S - 1) / S;
   FrameView f;
   f.L = a.L;
   f.starts = a.starts;
@@ -625,8 +654,7 @@ __global__ __launch_bounds__(64 * S, 8) void topo_propagate_kernel(
   f.steps = a.steps;
   f.exited = a.exited;
   f.counter = a.counter;
-  f.t0 = min(wave * tpw, ntiles);
-  f.t1 = min(f.t0 + tpw, ntiles);
+  wave_records(0, a.npad * Src::kRecPerSrc, wave, S, f.r0, f.r1);
   run_slots<S, Src>(ch, f, blockIdx.x * 64 + lane, part, esum, nsid, lane, wave);
 }
 
@@ -670,9 +698,6 @@ __global__ __launch_bounds__(64 * S, 8) void topo_batch_kernel(const float4* __r
     if (as >= a.n_assign) break;
     const int fr = a.assign[as];
     const int s0 = a.sl_off[fr];
-    const int c0 = a.chg_off[fr] / kTile;
-    const int ntiles = a.chg_off[fr + 1] / kTile - c0;
-    const int tpw = (ntiles + S - 1) / S;
     FrameView f;
     f.L = a.sl_off[fr + 1] - s0;
     f.starts = a.starts + 3LL * s0;
@@ -686,8 +711,7 @@ __global__ __launch_bounds__(64 * S, 8) void topo_batch_kernel(const float4* __r
     f.steps = a.steps + s0;
     f.exited = a.exited + s0;
     f.counter = a.sl_counter + fr;
-    f.t0 = c0 + min(wave * tpw, ntiles);
-    f.t1 = c0 + min(wave * tpw + tpw, ntiles);
+    wave_records(a.chg_off[fr], a.chg_off[fr + 1] - a.chg_off[fr], wave, S, f.r0, f.r1);
     run_slots<S, ChargeSrc>(ch, f, -1, part, esum, nsid, lane, wave);
   }
 }
