@@ -477,7 +477,7 @@ struct FrameView {
 };
 
 // Runs the workgroup's 64 slots until the frame's queue is exhausted.  Slot
-// state is identical in every wave: each wave sums its charge tiles for all
+// state is identical in every wave: each wave sums its charge slice for all
 // 64 slots, the partials are reduced in a fixed order through LDS, and all
 // waves apply the same FP64 update.  With S >= 4 waves, waves 0-2 reduce one
 // field component each into esum and the others wait at a second barrier:
