@@ -19,7 +19,8 @@
 //   2. propagate (persistent): a workgroup of S waves owns 64 streamline
 //      slots; wave s sums the field of charge slice s for all 64 slots (FP32
 //      per chunk of up to 64 charges, FP64 across chunks; the slices differ
-//      by at most four charges), the S partials are reduced in
+//      by at most four charges and leave out the trailing padding), the S
+//      partials are reduced in
 //      a fixed order through LDS and every wave applies the identical FP64
 //      position update.  A finished slot takes the next id from a global
 //      queue.  Results do not depend on which slot runs a streamline.
@@ -285,6 +286,33 @@ __device__ __forceinline__ void stream_recs(const float4* __restrict__ ch, int r
     ey += (double)fy;
     ez += (double)fz;
   }
+}
+
+// Records of the sources [rec0, rec0 + nrec) that precede the trailing
+// padding (rps records per source; padding sources sit at kPadCoord with a
+// zero charge or moment, so their terms are exactly 0), rounded up to a
+// multiple of four.  Padding is fewer than kTile sources at the end of a
+// block of a multiple of kTile sources; lane l of the calling wave checks
+// source nsrc - 64 + l.  Every lane of the wave must call it.
+__device__ __forceinline__ int used_records(const float4* __restrict__ ch, int rec0, int nrec,
+                                            int rps, int lane) {
+  const int nsrc = nrec / rps;
+  const int sidx = nsrc - 64 + lane;
+  bool real = true;
+  if (sidx >= 0) {
+    const float4 X = ch[rec0 + sidx * rps];
+    bool pad = X.x == kPadCoord && X.y == kPadCoord && X.z == kPadCoord;
+    if (rps == 1) {
+      pad = pad && X.w == 0.f;
+    } else {
+      const float4 M = ch[rec0 + sidx * rps + 1];
+      pad = pad && M.x == 0.f && M.y == 0.f && M.z == 0.f;
+    }
+    real = !pad;
+  }
+  const unsigned long long m = __ballot(real);
+  const int used = m ? (nsrc - 64 + (63 - __clzll(m)) + 1) * rps : 0;
+  return min((used + 3) & ~3, nrec);
 }
 
 // Splits n4 groups of four records (from record rec0) evenly over S waves:
@@ -654,7 +682,8 @@ __global__ __launch_bounds__(64 * S, 8) void topo_propagate_kernel(
   f.steps = a.steps;
   f.exited = a.exited;
   f.counter = a.counter;
-  wave_records(0, a.npad * Src::kRecPerSrc, wave, S, f.r0, f.r1);
+  const int nrec = used_records(ch, 0, a.npad * Src::kRecPerSrc, Src::kRecPerSrc, lane);
+  wave_records(0, __builtin_amdgcn_readfirstlane(nrec), wave, S, f.r0, f.r1);
   run_slots<S, Src>(ch, f, blockIdx.x * 64 + lane, part, esum, nsid, lane, wave);
 }
 
@@ -711,7 +740,8 @@ __global__ __launch_bounds__(64 * S, 8) void topo_batch_kernel(const float4* __r
     f.steps = a.steps + s0;
     f.exited = a.exited + s0;
     f.counter = a.sl_counter + fr;
-    wave_records(a.chg_off[fr], a.chg_off[fr + 1] - a.chg_off[fr], wave, S, f.r0, f.r1);
+    const int nrec = used_records(ch, a.chg_off[fr], a.chg_off[fr + 1] - a.chg_off[fr], 1, lane);
+    wave_records(a.chg_off[fr], __builtin_amdgcn_readfirstlane(nrec), wave, S, f.r0, f.r1);
     run_slots<S, ChargeSrc>(ch, f, -1, part, esum, nsid, lane, wave);
   }
 }
