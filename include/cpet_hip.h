@@ -104,6 +104,56 @@ int thread_operation_dipole(int n_dipoles, int n_iter, float step_size, const fl
                             const float *dimensions, const float *x, const float *mu,
                             float *ret);
 
+/* E[m] = k * sum_j Q[j] (x_0[m] - x[j]) / |x_0[m] - x[j]|^3 (no softening)
+ * for m < total_points; batch_size (> 0) is the reference's chunk length
+ * and does not change the result.  Runs as one launch of the unsoftened
+ * points kernel.  Replaces compute_batched_field,
+ * CPET/utils/math_module.c:374-403 (binding CPET/utils/c_ops.py:140-149,
+ * wrapper :265-279). */
+int compute_batched_field(int total_points, int batch_size, int n_charges, const float *x_0,
+                          const float *x, const float *Q, float *E);
+
+/* The remaining helpers the reference binding sets argtypes on
+ * (CPET/utils/c_ops.py:26-77).  They are off the hot path and run on the
+ * host, sequentially, in the element order and arithmetic types of the
+ * reference loops; no GPU is needed. */
+
+/* ret[r] = sum_{j = indptr[r]}^{indptr[r+1]-1} A[j] * B[indA[j]] over the
+ * indptrlen - 1 rows of a CSR matrix (FP64).  Replaces sparse_dot,
+ * CPET/utils/math_module.c:15-46 (binding CPET/utils/c_ops.py:26-36). */
+int sparse_dot(double *ret, const int *indptr, int indptrlen, const int *indA, int lenindA,
+               const double *A, int lenA, const double *B, int size_array);
+
+/* ret[i] = sum_j A[i*cols + j] * B[j] (FP64).  Replaces dot,
+ * CPET/utils/math_module.c:49-68 (binding CPET/utils/c_ops.py:44-50). */
+int dot(double *ret, const double *A, const double *B, int rows, int cols);
+
+/* ret = A + B (float32).  Replaces vecaddn, CPET/utils/math_module.c:124-132
+ * (binding CPET/utils/c_ops.py:37-42). */
+int vecaddn(float *ret, const float *A, const float *B, int lenA);
+
+/* result[k] = sum_i (k * Q[i] * r_mag[k][i]) R[k][i][:] with r_mag
+ * (batch,rows), Q (rows), R (batch,rows,3), result (batch,3), float32.
+ * Replaces einsum_operation_batch, CPET/utils/math_module.c:135-172
+ * (binding CPET/utils/c_ops.py:66-74). */
+int einsum_operation_batch(int batch, int rows, const float *r_mag, const float *Q,
+                           const float *R, float *result);
+
+/* einsum_operation_batch for one point: result (3).  Replaces
+ * einsum_operation, CPET/utils/math_module.c:175-209 (binding
+ * CPET/utils/c_ops.py:76-83). */
+int einsum_operation(int rows, const float *r_mag, const float *Q, const float *R,
+                     float *result);
+
+/* ret[k][i] = sum_j A[k][i][j], A (batch,rows,cols), float32.  Replaces
+ * einsum_ij_i_batch, CPET/utils/math_module.c:212-232 (binding
+ * CPET/utils/c_ops.py:58-64). */
+int einsum_ij_i_batch(int batch, int rows, int cols, const float *A, float *ret);
+
+/* ret[i] = sum_j A[i][j], A (rows,cols), float32.  Replaces einsum_ij_i,
+ * CPET/utils/math_module.c:235-252 (binding CPET/utils/c_ops.py:52-57). */
+int einsum_ij_i(int rows, int cols, const float *A, float *ret);
+
 /* ============ Part 2: batched / device-resident API ====================== */
 
 typedef struct cpet_ctx cpet_ctx;

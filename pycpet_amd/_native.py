@@ -64,6 +64,15 @@ SIGNATURES = {
     "calc_field_base": [P, P, c_int, P, P],
     "thread_operation": [c_int, c_int, c_float, P, P, P, P, P],
     "thread_operation_dipole": [c_int, c_int, c_float, P, P, P, P, P],
+    "compute_batched_field": [c_int, c_int, c_int, P, P, P, P],
+    # host helpers of the reference binding (no GPU)
+    "sparse_dot": [P, P, c_int, P, c_int, P, c_int, P, c_int],
+    "dot": [P, P, P, c_int, c_int],
+    "vecaddn": [P, P, P, c_int],
+    "einsum_operation_batch": [c_int, c_int, P, P, P, P],
+    "einsum_operation": [c_int, P, P, P, P],
+    "einsum_ij_i_batch": [c_int, c_int, c_int, P, P],
+    "einsum_ij_i": [c_int, c_int, P, P],
     # batched / device-resident
     "cpet_device_count": [P],
     "cpet_ctx_create": [c_int, P],

@@ -16,13 +16,23 @@ from typing import List, Optional, Sequence
 import numpy as np
 
 from . import _native as nat
-from .ops import TopoResult, _is_torch, _prec_flag, _ptr
+from .ops import TopoResult, _is_torch, _prec_flag, _ptr, dev_tensor
 
 
-def _cat(parts, dtype, torch_mode):
+def _cat(parts, dtype, torch_mode, name):
+    """Concatenates the frames' arrays flat.  Device tensors are checked
+    like the single-frame calls' (float32 data; integer step limits become
+    int32), so none is read as another type."""
     if torch_mode:
         import torch
 
+        kind = "i32" if dtype == np.int32 else "f32"
+        for p in parts:
+            if not _is_torch(p):
+                raise ValueError(f"{name}: mix of numpy arrays and torch tensors")
+        parts = [dev_tensor(p.contiguous(), kind, name) for p in parts]
+        if len({p.device for p in parts}) != 1:
+            raise ValueError(f"{name}: frames on different devices")
         return torch.cat([p.reshape(-1) for p in parts]).contiguous()
     return np.ascontiguousarray(np.concatenate([np.asarray(p, dtype=dtype).reshape(-1)
                                                 for p in parts]))
@@ -37,9 +47,11 @@ def batch_volume(frames: Sequence[dict], *, want_E: bool = True, want_phi: bool 
     nch = np.array([int(np.prod(f["q"].shape)) for f in frames], dtype=np.int32)
     dims = np.array([[len(a) for a in f["axes"]] for f in frames], dtype=np.int32).reshape(-1)
     M = [int(np.prod(dims[3 * i:3 * i + 3])) for i in range(F)]
-    x = _cat([f["x"] for f in frames], np.float32, tm)
-    q = _cat([f["q"] for f in frames], np.float32, tm)
-    axes = _cat([a for f in frames for a in f["axes"]], np.float32, tm)
+    x = _cat([f["x"] for f in frames], np.float32, tm, "x")
+    q = _cat([f["q"] for f in frames], np.float32, tm, "q")
+    axes = _cat([a for f in frames for a in f["axes"]], np.float32, tm, "axes")
+    if x.shape[0] != 3 * int(nch.sum()):
+        raise ValueError("x must hold 3 coordinates per charge of every frame")
     flags = _prec_flag(prec) | (nat.F_OUT_F64 if out_f64 else 0)
     tot = sum(M)
     if tm:
@@ -78,19 +90,19 @@ def batch_topo(frames: Sequence[dict], *, edges=None,
                                           if not _is_torch(f["dims"]) else
                                           f["dims"].detach().cpu().numpy().reshape(3)
                                           for f in frames], dtype=np.float32).reshape(-1))
-    x = _cat([f["x"] for f in frames], np.float32, tm)
-    q = _cat([f["q"] for f in frames], np.float32, tm)
-    starts = _cat([f["starts"] for f in frames], np.float32, tm)
-    n_iter = _cat([f["n_iter"] for f in frames], np.int32, tm)
+    x = _cat([f["x"] for f in frames], np.float32, tm, "x")
+    q = _cat([f["q"] for f in frames], np.float32, tm, "q")
+    starts = _cat([f["starts"] for f in frames], np.float32, tm, "starts")
+    n_iter = _cat([f["n_iter"] for f in frames], np.int32, tm, "n_iter")
     Lt = int(ns.sum())
+    if x.shape[0] != 3 * int(nch.sum()) or starts.shape[0] != 3 * Lt or n_iter.shape[0] != Lt:
+        raise ValueError("frame arrays do not match their charge / start counts")
     flags = 0
     if tm:
         import torch
 
         dev = x.device
         ctx.set_stream(torch.cuda.current_stream(dev).cuda_stream)
-        if n_iter.dtype != torch.int32:
-            n_iter = n_iter.to(torch.int32)
         feats = torch.empty((Lt, 2), dtype=torch.float64, device=dev)
         steps = torch.empty((Lt,), dtype=torch.int32, device=dev)
         exited = torch.empty((Lt,), dtype=torch.int32, device=dev)

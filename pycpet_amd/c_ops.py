@@ -97,6 +97,89 @@ class Math_ops:
             dims.ctypes.data, x.ctypes.data, mu.ctypes.data, res.ctypes.data))
         return res
 
+    # CPET/utils/c_ops.py:265-279 -> math_module.c:374-403
+    def compute_batch_field(self, x_0, x, Q, batch_size):
+        x_0 = _f32(x_0, 3)
+        x = _f32(x, 3)
+        Q = _f32(Q)
+        res = np.zeros_like(x_0, dtype="float32")
+        self._check(self.math.compute_batched_field(
+            int(x_0.shape[0]), int(batch_size), len(Q), x_0.ctypes.data, x.ctypes.data,
+            Q.ctypes.data, res.ctypes.data))
+        return res
+
+    # Host helpers (CPET/utils/c_ops.py:161-248 -> math_module.c:15-252).
+    # The reference wrappers hand float32 buffers to the FP64 C arguments of
+    # sparse_dot and dot; here the arrays are converted to the types the C
+    # functions read, and the row results come back as float64.
+    def sparse_dot(self, A, B):
+        """CSR matrix A (scipy.sparse) times vector B: one value per row."""
+        indptr = np.ascontiguousarray(A.indptr, dtype=np.int32)
+        indices = np.ascontiguousarray(A.indices, dtype=np.int32)
+        data = np.ascontiguousarray(A.data, dtype=np.float64)
+        b = np.ascontiguousarray(B, dtype=np.float64).ravel()
+        res = np.zeros(max(len(indptr) - 1, 0), dtype=np.float64)
+        self._check(self.math.sparse_dot(res.ctypes.data, indptr.ctypes.data, len(indptr),
+                                         indices.ctypes.data, len(indices), data.ctypes.data,
+                                         len(data), b.ctypes.data, len(b)))
+        return res
+
+    def dot(self, A, B):
+        """Dense matrix A (rows, cols) times vector B (cols)."""
+        a = np.ascontiguousarray(A, dtype=np.float64)
+        a = a.reshape(a.shape[0], -1)
+        b = np.ascontiguousarray(B, dtype=np.float64).ravel()
+        if a.shape[1] != b.size:
+            raise ValueError(f"A has {a.shape[1]} columns but B has {b.size} entries")
+        res = np.zeros(a.shape[0], dtype=np.float64)
+        self._check(self.math.dot(res.ctypes.data, a.ctypes.data, b.ctypes.data, a.shape[0],
+                                  a.shape[1]))
+        return res
+
+    def vecaddn(self, A, B):
+        a, b = _f32(A), _f32(B)
+        if a.size != b.size:
+            raise ValueError("A and B must have the same length")
+        res = np.zeros(a.size, dtype="float32")
+        self._check(self.math.vecaddn(res.ctypes.data, a.ctypes.data, b.ctypes.data, a.size))
+        return res
+
+    def einsum_ij_i(self, A):
+        a = np.ascontiguousarray(A, dtype=np.float32)
+        res = np.zeros((a.shape[0]), dtype="float32")
+        self._check(self.math.einsum_ij_i(a.shape[0], a.shape[1], a.ctypes.data,
+                                          res.ctypes.data))
+        return res
+
+    def einsum_ij_i_batch(self, A):
+        a = np.ascontiguousarray(A, dtype=np.float32)
+        res = np.zeros((a.shape[0], a.shape[1]), dtype="float32")
+        self._check(self.math.einsum_ij_i_batch(a.shape[0], a.shape[1], a.shape[2],
+                                                a.ctypes.data, res.ctypes.data))
+        # the reference reshapes (not transposes) the (batch, rows) result
+        return res.reshape(res.shape[1], res.shape[0])
+
+    def einsum_operation(self, R, r_mag, Q):
+        r_mag = _f32(r_mag)
+        R = _f32(R, 3)
+        Q = _f32(Q)
+        if not (R.shape[0] == r_mag.size == Q.size):
+            raise ValueError("R, r_mag and Q must describe the same charges")
+        res = np.zeros(3, dtype="float32")
+        self._check(self.math.einsum_operation(Q.size, r_mag.ctypes.data, Q.ctypes.data,
+                                               R.ctypes.data, res.ctypes.data))
+        return res
+
+    def einsum_operation_batch(self, R, r_mag, Q, batch_size):
+        Q = _f32(Q)
+        r_mag = np.ascontiguousarray(r_mag, dtype=np.float32).reshape(batch_size, Q.size)
+        R = np.ascontiguousarray(R, dtype=np.float32).reshape(batch_size, Q.size, 3)
+        res = np.zeros((batch_size, 3), dtype="float32")
+        self._check(self.math.einsum_operation_batch(batch_size, Q.size, r_mag.ctypes.data,
+                                                     Q.ctypes.data, R.ctypes.data,
+                                                     res.ctypes.data))
+        return res
+
     # Batched forms (no per-point / per-streamline FFI loop).
     def esp_points(self, x_0, x, Q):
         """phi at every row of x_0 in one launch (float32)."""

@@ -81,9 +81,10 @@ class calculator:
                                      options, seltype="center")
             self.center = calculate_center(pos, method=options["center"]["method"])
         else:
-            raise ValueError("center must be a list or dict")
+            raise ValueError("option 'center' takes a coordinate list or an atom selection "
+                             f"dict, not {type(options['center']).__name__}")
         if "x" not in options.keys():
-            self.log.info("No x specified, calculating in input file reference frame. Ignoring y...")
+            self.log.info("option 'x' absent: the box keeps the input frame's axes and 'y' is not read")
             compute_y = False
         elif type(options["x"]) == list:
             self.x_vec_pt = np.array(options["x"])
@@ -94,7 +95,8 @@ class calculator:
             self.x_vec_pt = calculate_center(pos, method=options["x"]["method"])
             compute_y = True
         else:
-            raise ValueError("x must be a list or dict")
+            raise ValueError("option 'x' takes a coordinate list or an atom selection dict, "
+                             f"not {type(options['x']).__name__}")
         if compute_y:
             if type(options["y"]) == list:
                 self.y_vec_pt = np.array(options["y"])
@@ -103,7 +105,8 @@ class calculator:
                                          self.chains, options, seltype="y")
                 self.y_vec_pt = calculate_center(pos, method=options["y"]["method"])
             else:
-                raise ValueError("Since you have provided x, y must be a list or dict")
+                raise ValueError("option 'y' must be a coordinate list or an atom selection dict "
+                                 "whenever 'x' is given")
 
         self.x_copy = deepcopy(self.x)
         self.residue_number_copy = deepcopy(self.residue_number)

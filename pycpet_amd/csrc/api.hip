@@ -668,6 +668,20 @@ int compute_looped_field(int total_points, int n_charges, const float* x_0,
                            1, 0u);
 }
 
+int compute_batched_field(int total_points, int batch_size, int n_charges, const float* x_0,
+                          const float* x, const float* Q, float* E) {
+  // batch_size only chunks the reference's loop (math_module.c:375-377); a
+  // non-positive value would never advance it, so it is rejected here
+  if (batch_size <= 0) {
+    set_error("compute_batched_field: batch_size must be > 0 (got %d)", batch_size);
+    return CPET_ERR_ARG;
+  }
+  cpet_ctx* c = nullptr;
+  int rc = default_ctx(&c);
+  if (rc) return rc;
+  return cpet_field_points(c, total_points, n_charges, x_0, x, Q, E, nullptr, 0, 0u);
+}
+
 int calc_esp_base(float* ESP, const float* x_init, int n_charges,
                   const float* x, const float* Q) {
   if (!ESP) {

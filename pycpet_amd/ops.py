@@ -33,17 +33,59 @@ def _ptr(a) -> int:
     return a.ctypes.data
 
 
-def _torch_dev_args(ctx: nat.Context, *tensors):
+def dev_tensor(t, kind: str, name: str):
+    """Checks a torch tensor handed to the library as a raw device pointer.
+
+    kind "f32": coordinates, charges, axes, starts, moments — must be
+    float32 (a float64 or float16 tensor would be read as the wrong type, or
+    past its end), else ValueError.  kind "i32": step limits — any integer
+    dtype, converted to int32 (an int64 tensor from np.asarray of Python
+    ints is common); a floating dtype is a ValueError.  The tensor must live
+    on a GPU and be contiguous.  Returns the tensor to pass (a converted
+    copy for integer step limits of another width)."""
     import torch
 
-    for t in tensors:
-        if t is None:
-            continue
-        if not t.is_cuda:
-            raise ValueError("torch inputs must be device tensors")
-        if not t.is_contiguous():
-            raise ValueError("torch inputs must be contiguous")
+    if kind == "f32":
+        if t.dtype != torch.float32:
+            raise ValueError(f"{name} must be a float32 tensor (got {t.dtype}); "
+                             f"convert it with .float()")
+    elif kind == "i32":
+        if t.dtype.is_floating_point or t.dtype.is_complex or t.dtype == torch.bool:
+            raise ValueError(f"{name} must be an integer tensor (got {t.dtype})")
+    else:
+        raise ValueError(f"unknown tensor kind {kind!r}")
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device tensor (got one on {t.device})")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if kind == "i32" and t.dtype != torch.int32:
+        if t.numel() and (int(t.max()) > 2**31 - 1 or int(t.min()) < -2**31):
+            raise ValueError(f"{name} holds values outside int32")
+        t = t.to(torch.int32)
+    return t
+
+
+def _rows3(a, name: str) -> int:
+    """Rows of an (n,3) array or tensor (a flat one of 3n values counts as
+    n rows); anything else is a ValueError, so no kernel reads past it."""
+    n = a.numel() if _is_torch(a) else a.size
+    if n % 3 or (a.ndim == 2 and a.shape[1] != 3) or a.ndim > 2:
+        raise ValueError(f"{name} must hold rows of 3 values (got shape {tuple(a.shape)})")
+    return n // 3
+
+
+def _torch_dev_args(ctx: nat.Context, *tensors):
+    """Validates float32 device tensors (see dev_tensor) and points the
+    context at torch's current stream on their device."""
+    import torch
+
+    for i, t in enumerate(tensors):
+        if t is not None:
+            dev_tensor(t, "f32", f"tensor argument {i}")
     dev = tensors[0].device
+    for t in tensors:
+        if t is not None and t.device != dev:
+            raise ValueError("all tensor arguments must be on one device")
     ctx.set_stream(torch.cuda.current_stream(dev).cuda_stream)
     return dev
 
@@ -85,8 +127,8 @@ def field_grid(ax, ay, az, x, q, *, want_E: bool = True, want_phi: bool = False,
         E = np.empty((M, 3), dtype=odt) if want_E else None
         phi = np.empty((M,), dtype=odt) if want_phi else None
         N = q.size
-    if x.shape[0] != N:
-        raise ValueError(f"x has {x.shape[0]} rows but q has {N} charges")
+    if _rows3(x, "x") != N:
+        raise ValueError(f"x has {_rows3(x, 'x')} rows but q has {N} charges")
     nat.check(ctx.lib.cpet_field_grid(
         ctx.handle, nx, ny, nz, _ptr(ax), _ptr(ay), _ptr(az), N, _ptr(x), _ptr(q),
         _ptr(E) if E is not None else None, _ptr(phi) if phi is not None else None,
@@ -104,7 +146,7 @@ def field_points(pts, x, q, *, want_E: bool = True, want_phi: bool = False,
         import torch
 
         dev = _torch_dev_args(ctx, pts, x, q)
-        M = pts.shape[0]
+        M = _rows3(pts, "pts")
         odt = torch.float64 if out_f64 else torch.float32
         E = torch.empty((M, 3), dtype=odt, device=dev) if want_E else None
         phi = torch.empty((M,), dtype=odt, device=dev) if want_phi else None
@@ -119,8 +161,8 @@ def field_points(pts, x, q, *, want_E: bool = True, want_phi: bool = False,
         E = np.empty((M, 3), dtype=odt) if want_E else None
         phi = np.empty((M,), dtype=odt) if want_phi else None
         N = q.size
-    if x.shape[0] != N:
-        raise ValueError(f"x has {x.shape[0]} rows but q has {N} charges")
+    if _rows3(x, "x") != N:
+        raise ValueError(f"x has {_rows3(x, 'x')} rows but q has {N} charges")
     nat.check(ctx.lib.cpet_field_points(
         ctx.handle, M, N, _ptr(pts), _ptr(x), _ptr(q),
         _ptr(E) if E is not None else None, _ptr(phi) if phi is not None else None,
@@ -165,8 +207,9 @@ def topo(starts, n_iter, h: float, dims, x, q, *, edges=None, integrator: str = 
     if _is_torch(x):
         import torch
 
-        dev = _torch_dev_args(ctx, starts, n_iter, x, q)
-        L = starts.shape[0]
+        dev = _torch_dev_args(ctx, starts, x, q)
+        n_iter = dev_tensor(n_iter, "i32", "n_iter").reshape(-1)
+        L = _rows3(starts, "starts")
         feats = torch.empty((L, 2), dtype=torch.float64, device=dev)
         steps = torch.empty((L,), dtype=torch.int32, device=dev)
         exited = torch.empty((L,), dtype=torch.int32, device=dev)
@@ -196,8 +239,8 @@ def topo(starts, n_iter, h: float, dims, x, q, *, edges=None, integrator: str = 
         N = q.size
     if n_iter.shape[0] != L:
         raise ValueError("n_iter must have one entry per start point")
-    if x.shape[0] != N:
-        raise ValueError(f"x has {x.shape[0]} rows but q has {N} charges")
+    if _rows3(x, "x") != N:
+        raise ValueError(f"x has {_rows3(x, 'x')} rows but q has {N} charges")
     nb_d = (ed.shape[0] - 1) if ed is not None else 0
     nb_c = (ec.shape[0] - 1) if ec is not None else 0
     nat.check(ctx.lib.cpet_topo(
@@ -223,13 +266,14 @@ def topo_dipole(starts, n_iter, h: float, dims, x, mu, *, physical: bool = False
     if _is_torch(x):
         import torch
 
-        dev = _torch_dev_args(ctx, starts, n_iter, x, mu)
-        L = starts.shape[0]
+        dev = _torch_dev_args(ctx, starts, x, mu)
+        n_iter = dev_tensor(n_iter, "i32", "n_iter").reshape(-1)
+        L = _rows3(starts, "starts")
         feats = torch.empty((L, 2), dtype=torch.float64, device=dev)
         steps = torch.empty((L,), dtype=torch.int32, device=dev)
         exited = torch.empty((L,), dtype=torch.int32, device=dev)
         flags |= nat.F_DEVICE | nat.F_NO_SYNC
-        N = x.shape[0]
+        N = _rows3(x, "x")
     else:
         starts = _host_f32(starts, 3)
         n_iter = np.ascontiguousarray(n_iter, dtype=np.int32).ravel()
@@ -242,7 +286,7 @@ def topo_dipole(starts, n_iter, h: float, dims, x, mu, *, physical: bool = False
         N = x.shape[0]
     if n_iter.shape[0] != L:
         raise ValueError("n_iter must have one entry per start point")
-    if tuple(mu.shape) != (N, 3):
+    if _rows3(mu, "mu") != N:
         raise ValueError("mu must be (N, 3) like x")
     nat.check(ctx.lib.cpet_topo_dipole(
         ctx.handle, N, L, _ptr(starts), _ptr(n_iter), ctypes.c_float(h), _ptr(dims), _ptr(x),

@@ -113,6 +113,40 @@ def test_dropin_entry_points(gpu_ctx, oracle_mod):
     np.testing.assert_allclose(Eb, pf, rtol=1e-6)
 
 
+def test_compute_batched_field_through_reference_binding(gpu_ctx, oracle_mod):
+    """compute_batched_field bound with the reference's argtypes (ndpointer
+    float32, CPET/utils/c_ops.py:140-149) equals the unsoftened FP64 field
+    and, within its FP32 rounding, the reference C of the same name."""
+    import ctypes
+
+    from pycpet_amd import _native as nat
+    from pycpet_amd.c_ops import Math_ops
+    from test_abi import _bind_like_reference
+
+    g = np.load(GOLDEN / "ref_volume_500.npz")
+    pts = mesh([g["ax"], g["ay"], g["az"]])
+    lib, _ = _bind_like_reference(nat.LIB_PATH)
+    E = np.zeros_like(pts)
+    x = np.ascontiguousarray(g["x"], np.float32)
+    q = np.ascontiguousarray(g["q"], np.float32).ravel()
+    lib.compute_batched_field(pts.shape[0], 100, q.size, pts, x, q, E)
+    E64, _ = oracle_mod.field_f64(pts, x, q, soften=False)
+    assert rel_err(E, E64)[0] <= TOL
+    Eb = Math_ops().compute_batch_field(pts, x, q.reshape(-1, 1), 37)
+    np.testing.assert_array_equal(Eb, E)  # the chunk length changes nothing
+    if oracle_mod.REF_SO.exists():
+        ref = ctypes.CDLL(str(oracle_mod.REF_SO))
+        ref.compute_batched_field.restype = None
+        Er = np.zeros_like(pts)
+        P = ctypes.c_void_p
+        ref.compute_batched_field(ctypes.c_int(pts.shape[0]), ctypes.c_int(100),
+                                  ctypes.c_int(q.size), pts.ctypes.data_as(P),
+                                  x.ctypes.data_as(P), q.ctypes.data_as(P), Er.ctypes.data_as(P))
+        assert rel_err(E, Er)[0] < 5e-5  # the reference is FP32 -ffast-math
+    assert nat.load().compute_batched_field(4, 0, q.size, pts.ctypes.data, x.ctypes.data,
+                                            q.ctypes.data, E.ctypes.data) == nat.CPET_ERR_ARG
+
+
 def test_edge_cases(gpu_ctx, oracle_mod):
     from pycpet_amd import ops
     from pycpet_amd._native import CpetError

@@ -8,6 +8,7 @@ distance = steps * h, zero curvature), the continuous field-line curvature
 difference converges to it as h -> 0), and the n_iter = 0 / 1 edge cases.
 """
 import numpy as np
+import pytest
 
 from conftest import GOLDEN
 
@@ -260,3 +261,64 @@ def test_rk4_agrees_with_euler_as_h_shrinks(oracle_mod):
     *_, pe = oracle_mod.topo_f64(start, np.array([20000], np.int32), 0.00001, dims, x, q,
                                  want_points=True)
     assert np.linalg.norm(pr[0, 3] - pe[0, 3]) < 1e-5
+
+
+def test_f32_port_matches_reference_golden(oracle_mod):
+    """The FP32 port of thread_operation (oracle_topo_f32, the restated CPU
+    baseline) against the reference C's own FP32 outputs kept in
+    tests/golden/ref_topo_200.npz: both walk in FP32, so distances agree to
+    a few float32 ulps and curvatures to FP32 finite-difference noise."""
+    g = np.load(GOLDEN / "ref_topo_200.npz")
+    f32, s32 = oracle_mod.topo_f32(g["starts"], g["n_iter"], float(g["h"]), g["dims"], g["x"],
+                                   g["q"])
+    ref = g["ret"].astype(float)
+    rd = np.abs(ref[:, 0] - f32[:, 0]) / np.maximum(ref[:, 0], 1e-12)
+    assert rd.max() < 1e-5, rd.max()
+    rc = np.abs(ref[:, 1] - f32[:, 1]) / np.abs(ref[:, 1])
+    assert np.median(rc) < 1e-2 and np.percentile(rc, 90) < 3e-2, (np.median(rc),)
+    # the same walk as the FP64 oracle: identical step counts on this case
+    _, s64, _ = oracle_mod.topo_f64(g["starts"], g["n_iter"], float(g["h"]), g["dims"], g["x"],
+                                    g["q"])
+    np.testing.assert_array_equal(s32, s64)
+
+
+def test_f32_port_matches_live_reference(oracle_mod):
+    """A fresh seeded case at h = 1e-3 through the reference's compiled
+    thread_operation (oracle/_ref), one call per streamline."""
+    if not oracle_mod.Ref.available():
+        pytest.skip("reference C not built here")
+    rng = np.random.default_rng(21)
+    d = rng.standard_normal((300, 3))
+    x = (d / np.linalg.norm(d, axis=1, keepdims=True) * rng.uniform(3.0, 12.0, (300, 1)))
+    x = x.astype(np.float32)
+    q = rng.normal(0, 0.4, 300).astype(np.float32)
+    starts = rng.uniform(-0.9, 0.9, (48, 3)).astype(np.float32)
+    n_iter = rng.integers(1, 1500, 48).astype(np.int32)
+    dims = np.full(3, 1.0, np.float32)
+    f32, _ = oracle_mod.topo_f32(starts, n_iter, 1e-3, dims, x, q)
+    want = oracle_mod.Ref().topo(starts, n_iter, 1e-3, dims, x, q).astype(float)
+    rd = np.abs(want[:, 0] - f32[:, 0]) / np.maximum(want[:, 0], 1e-12)
+    assert np.mean(rd < 1e-4) >= 0.95, np.mean(rd < 1e-4)
+
+
+def test_f32_port_radial_lines_and_zero_steps(oracle_mod):
+    """Closed forms for the FP32 port: radial straight lines from one charge
+    stop at the exact exit step away from ties, with distance = steps * h to
+    float32 rounding and curvature ~ 0; n_iter = 0 gives distance 0."""
+    rng = np.random.default_rng(7)
+    dims = np.array([0.6, 0.5, 0.4], np.float32)
+    h = 0.01
+    c = np.array([3.0, 0.5, -0.4])
+    starts = rng.uniform(-0.35, 0.35, (64, 3)).astype(np.float32)
+    n_iter = rng.integers(1, 200, 64).astype(np.int32)
+    n_iter[:4] = 0
+    f32, s32 = oracle_mod.topo_f32(starts, n_iter, h, dims, c[None].astype(np.float32),
+                                   np.array([0.8], np.float32))
+    exp_steps, clean, _ = _radial_expect(starts, c.astype(np.float32).astype(float), 1.0,
+                                         dims.astype(float), float(np.float32(h)), n_iter)
+    clean &= n_iter > 0
+    assert clean.sum() > 50
+    np.testing.assert_array_equal(s32[clean], exp_steps[clean])
+    np.testing.assert_allclose(f32[clean, 0], s32[clean] * float(np.float32(h)), rtol=1e-5)
+    assert np.all(f32[:4, 0] == 0.0) and np.all(s32[:4] == 0)
+    assert np.abs(f32[:, 1]).max() < 0.5  # FP32 finite differences of straight lines
