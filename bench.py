@@ -4,9 +4,13 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
 One "step" is one full pass of the streamline hot path (order, propagate,
-FP64 epilogue) over 100,000 starts with inputs already resident in HBM:
-5,000 charges (35 A ball, 3 A exclusion around the +-1.5 A box), h = 0.001 A,
-Euler, n_iter = randint(1, max_steps) (seed 42) capped at 2,000.
+FP64 epilogue with the 20 x 20 feature histogram) over 100,000 starts with
+inputs already resident in HBM: 5,000 charges (35 A ball, 3 A exclusion
+around the +-1.5 A box), h = 0.001 A, Euler, n_iter = randint(1, max_steps)
+(seed 42) capped at 2,000.  The histogram's range is the min / max of the
+FP64 features of the same streamlines (one untimed FP64 run first), as the
+reference's histogram step takes it from the data (make_histograms,
+CPET/utils/calculator.py:702-707).
 
 Prints ONE JSON line.  `value` is charge-point pair-evals/s, counted as
 (Euler steps + 4 curvature steps) x charges per streamline, as the reference
@@ -16,11 +20,15 @@ the stream it runs on: 18 FLOP per charge-point pair x charges x Euler steps
 per launch / the kernel's mean duration, against the FP32 vector peak.
 `cpu_baseline` times the reference's own thread_operation (built from its
 source under oracle/_ref) on a bounded seeded sample of the same streamlines
-on the host cores, or the oracle's FP32 port when that build is absent.
+on the host cores, or the oracle's FP32 port when that build is absent; its
+`volume` and `esp` entries time the reference's compute_looped_field and
+calc_esp_base on one core (the 2,000-charge 21^3 grid in full and a 21^3
+subset of the 30,000-charge 101^3 grid; ESP with and without the per-point
+Python loop).
 Secondary lines: `volume_30k` (the 101^3 grid over 30,000 clustered
 charges) and `volume_esp_30k` (the same grid, potential only; both
---no-volume), `topo_f64` (the same streamlines with FP64 field sums, and the
-FP32 path's match fraction and feature errors against it; --no-f64) and
+--no-volume), `topo_f64` (the FP64 run: its speed and the FP32 path's match
+fraction and feature errors against it; --no-f64 drops the line) and
 `topo_rk4` (the same streamlines integrated with RK4, four field
 evaluations per step; --no-rk4).
 """
@@ -95,9 +103,43 @@ def cpu_baseline(case, gpu_steps, seconds: float, cores: int):
         "cores": cores,
         "kind": kind,
         "sample": f"{len(done)} of {case.L} streamlines (seeded), {dt:.1f} s wall; "
-                  "pair-evals counted with the GPU's per-streamline step counts",
+                  "pair-evals counted with the GPU's per-streamline step counts; "
+                  "one reference call per streamline from a thread pool (ctypes releases "
+                  "the GIL; the reference fans the same calls over a process Pool)",
         "streamlines_per_s": len(done) / dt,
     }
+
+
+def cpu_baseline_grids():
+    """Reference compute_looped_field and calc_esp_base on one core: the
+    2,000-charge 21^3 grid in full, and the 21^3 subset (every 5th point per
+    axis) of the 101^3 grid over 30,000 clustered charges.  ESP is timed with
+    the reference's per-point Python loop and with the same calls from C."""
+    from oracle import oracle as O
+    from pycpet_amd import workloads as W
+
+    if not O.Ref.available():
+        return None
+    ref = O.Ref()
+    c0 = W.volume_2k()
+    c2 = W.volume_30k()
+    sub = [a[::5] for a in c2.axes]
+    cases = {"volume_2k_21^3": (c0.mesh_points(), c0.x, c0.q),
+             "volume_30k_21^3_of_101^3": (
+                 np.stack(np.meshgrid(*sub, indexing="ij"), -1).reshape(-1, 3).astype(np.float32),
+                 c2.x, c2.q)}
+    out = {"volume": {}, "esp_python_loop": {}, "esp_c_loop": {}}
+    for name, (pts, x, q) in cases.items():
+        pairs = float(pts.shape[0]) * q.size
+        for key, fn in (("volume", ref.compute_looped_field),
+                        ("esp_python_loop", ref.esp_grid_python_loop),
+                        ("esp_c_loop", ref.esp_grid_c_loop)):
+            t0 = time.perf_counter()
+            fn(pts, x, q)
+            dt = time.perf_counter() - t0
+            out[key][name] = {"pair_evals_per_s": pairs / dt, "seconds": dt,
+                              "points": int(pts.shape[0]), "charges": int(q.size), "cores": 1}
+    return out
 
 
 def main() -> int:
@@ -128,8 +170,20 @@ def main() -> int:
     ctx = nat.Context(0)
     ctx.set_profiling(True)
 
+    # the FP64 run (untimed): the histogram range, and the topo_f64 line
+    ops.topo(starts, n_iter, case.h, dims, x, q, prec="f64", ctx=ctx)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    r64 = ops.topo(starts, n_iter, case.h, dims, x, q, prec="f64", ctx=ctx)
+    k64 = ctx.phase_times(3)[1]
+    torch.cuda.synchronize()
+    w64 = time.perf_counter() - t3
+    f64 = r64.feats.cpu().numpy()
+    edges = (torch.from_numpy(np.linspace(f64[:, 0].min(), f64[:, 0].max(), 21)).to(dev),
+             torch.from_numpy(np.linspace(f64[:, 1].min(), f64[:, 1].max(), 21)).to(dev))
+
     for _ in range(args.warmup):
-        r = ops.topo(starts, n_iter, case.h, dims, x, q, ctx=ctx)
+        r = ops.topo(starts, n_iter, case.h, dims, x, q, edges=edges, ctx=ctx)
     torch.cuda.synchronize()
 
     prop_ms = []
@@ -137,12 +191,18 @@ def main() -> int:
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        r = ops.topo(starts, n_iter, case.h, dims, x, q, ctx=ctx)
+        r = ops.topo(starts, n_iter, case.h, dims, x, q, edges=edges, ctx=ctx)
         ph = ctx.phase_times(3)  # waits for this step's last event
         prop_ms.append(ph[1])
         epi_ms.append(ph[2])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    hist = r.hist.cpu().numpy()
+    fr = r.feats.cpu().numpy().astype(np.float32).astype(np.float64)
+    hn, _, _ = np.histogram2d(fr[:, 0], fr[:, 1], bins=[e.cpu().numpy() for e in edges])
+    if not np.array_equal(hist, hn):
+        print("histogram differs from numpy.histogram2d of the features", file=sys.stderr)
+        return 1
 
     steps_np = r.steps.cpu().numpy()
     euler_steps = int(steps_np.astype(np.int64).sum())
@@ -168,7 +228,9 @@ def main() -> int:
         "config": {"workload": "topo_5k_charges_100k_streamlines", "charges": case.N,
                    "streamlines": case.L, "h": case.h, "box_half_width": 1.5,
                    "n_iter": "randint(1, 5196) seed 42, capped at 2000",
-                   "integrator": "euler (reference)", "euler_steps": euler_steps},
+                   "integrator": "euler (reference)", "euler_steps": euler_steps,
+                   "histogram": "20x20 (distance, mean curvature), range = min/max of the FP64 "
+                                "features, counted in the epilogue"},
         "streamlines_per_s": case.L * args.steps / wall,
         "roofline": {
             "bound": "valu",
@@ -236,15 +298,7 @@ def main() -> int:
         # the same streamlines with FP64 field sums: its speed, and the FP32
         # path's error against it over all 100,000 streamlines (the FP64 mode
         # is within 4e-15 of the CPU FP64 oracle on the parity tests)
-        ops.topo(starts, n_iter, case.h, dims, x, q, prec="f64", ctx=ctx)
-        torch.cuda.synchronize()
-        t3 = time.perf_counter()
-        r64 = ops.topo(starts, n_iter, case.h, dims, x, q, prec="f64", ctx=ctx)
-        k64 = ctx.phase_times(3)[1]
-        torch.cuda.synchronize()
-        w64 = time.perf_counter() - t3
         s64 = r64.steps.cpu().numpy()
-        f64 = r64.feats.cpu().numpy()
         f32 = r.feats.cpu().numpy()
         m = (steps_np == s64) & (r.exited.cpu().numpy() == r64.exited.cpu().numpy())
         dd = np.abs(f32[m, 0] - f64[m, 0]) / np.maximum(np.abs(f64[m, 0]), 1e-12)
@@ -283,8 +337,14 @@ def main() -> int:
         }
 
     if not args.no_cpu_baseline and int(os.environ.get("RANK", "0")) == 0:
-        cores = min(16, os.cpu_count() or 1)
+        avail = len(os.sched_getaffinity(0))
+        cores = min(16, avail)
         out["cpu_baseline"] = cpu_baseline(case, steps_np, args.cpu_seconds, cores)
+        out["cpu_baseline"]["host_nproc"] = os.cpu_count()
+        out["cpu_baseline"]["host_cpus_usable"] = avail
+        grids = cpu_baseline_grids()
+        if grids:
+            out["cpu_baseline"].update(grids)
     else:
         out["cpu_baseline"] = None
 

@@ -380,6 +380,22 @@ int cpet_pdb_parse(const char *path, int n_atoms, int get_charges, double *xyz, 
                    int32_t *atom_number, int32_t *resnum, char *atom_type, char *resname,
                    char *chain);
 
+/* Parses the n_atoms ATOM/HETATM records of a PQR file as
+ * CPET/utils/io.py:478-571 (parse_pqr) does: whitespace tokens give the
+ * serial, atom name, residue name and residue number (a first token longer
+ * than 6 characters holds the serial from its 7th character and shifts the
+ * rest by one; a residue-number token longer than 4 characters is replaced
+ * by the previous token without its first character); residue numbers above
+ * 999 shift the coordinate columns [30:38] [38:46] [46:54] and the charge
+ * column [54:61] right by (digits - 3).  Outputs: xyz[3*n_atoms],
+ * q[n_atoms], atom_number, resnum, and atom_type / resname in `width`-byte
+ * NUL-padded slots.  Count the records with cpet_pdb_scan (same rule).
+ * Returns CPET_ERR_ARG, naming the line, for a record Python would reject
+ * and for one this parser does not read exactly as Python would (non-ASCII
+ * bytes, digit-group underscores, a token of width or more bytes). */
+int cpet_pqr_parse(const char *path, int n_atoms, int width, double *xyz, double *q,
+                   int32_t *atom_number, int32_t *resnum, char *atom_type, char *resname);
+
 /* Writes an optional header then rows x cols values, byte-identical to
  * numpy.savetxt(path, data, fmt) for fmt 0 = "%.3f" (the volume .dat rows,
  * CPET/utils/io.py:97-101) and fmt 1 = "%.18e" (the .top histogram,

@@ -210,6 +210,43 @@ class Ref:
             out[i] = one[0]
         return out
 
+    def esp_grid_python_loop(self, pts, x, q):
+        """compute_ESP_on_grid's loop (CPET/utils/calculator.py:466-475) with
+        the reference wrapper's calling pattern: per point a fresh (1,)
+        float32 result, an ndpointer-checked calc_esp_base call
+        (CPET/utils/c_ops.py:326-341), the centre test and the float16 cast
+        at the end.  The CPU baseline of the ESP path with its FFI loop."""
+        import numpy.ctypeslib as npct
+
+        fn = self.L.calc_esp_base
+        f1 = npct.ndpointer(dtype=np.float32, ndim=1, flags="C")
+        f2 = npct.ndpointer(dtype=np.float32, ndim=2, flags="C")
+        fn.argtypes = [f1, f1, _i, f2, f1]
+        try:
+            pts, x, q = _f32(pts, 3), _f32(x, 3), _f32(q)
+            out = np.zeros((pts.shape[0], 1), dtype=float)
+            qr = q.reshape(len(q))
+            for i, x_0 in enumerate(pts):
+                res = np.zeros(1, dtype="float32")
+                fn(res, x_0, len(q), x, qr)
+                out[i] = res
+                if x_0[0] == 0 and x_0[1] == 0 and x_0[2] == 0:
+                    pass  # the reference logs the centre value here
+            return np.concatenate((pts, out), axis=1).astype(np.half)
+        finally:
+            fn.argtypes = [_P, _P, _i, _P, _P]
+
+    def esp_grid_c_loop(self, pts, x, q):
+        """The same per-point calc_esp_base calls from a C loop
+        (oracle/ref_loops.c): the ESP baseline without the Python loop."""
+        pts, x, q = _f32(pts, 3), _f32(x, 3), _f32(q)
+        fn = self.L.ref_esp_points
+        fn.argtypes = [_i, _P, _i, _P, _P, _P]
+        fn.restype = None
+        out = np.zeros(pts.shape[0], np.float32)
+        fn(pts.shape[0], _p(pts), q.size, _p(x), _p(q), _p(out))
+        return out
+
     def calc_field(self, pt, x, q):
         pt, x, q = _f32(pt), _f32(x, 3), _f32(q)
         E = np.zeros(3, np.float32)

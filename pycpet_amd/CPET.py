@@ -11,11 +11,21 @@ the hot path (cluster, pca, visualize, box_check) raise NotImplementedError.
 
 "topo" / "topo_GPU" run up to `frames_per_batch` unfinished frames (an
 option of this build, default 16; 1 gives the reference's one frame per
-call) through one batched streamline launch and then write their .top
-files.  The batch computes each frame exactly as the per-frame call does, so
-the files are the same; frames that alone cannot fill the GPU run together.
-With the option integrator = "rk4" (an extension; the reference integrates
-with Euler) frames run one per call.
+call) through one batched streamline launch and write their .top files as
+soon as the launch's results are back.  The batch computes each frame
+exactly as the per-frame call does, so the files are the same; frames that
+alone cannot fill the GPU run together.  With the option integrator = "rk4"
+(an extension; the reference integrates with Euler) frames run one per call.
+
+Crash and sharing behaviour.  Each .top is written to a temporary name and
+renamed into place, so a crash never leaves a partial file that a rerun
+would take for a finished frame.  A crash loses the frames of the group in
+flight: at most `frames_per_batch` frames (the reference, which writes after
+every frame, CPET/source/CPET.py:119-123, loses at most one).  Several
+processes may share one output directory as with the reference: before a
+group is launched, frames whose .top appeared meanwhile are dropped from it,
+so a frame is computed twice only when two processes launch it at the same
+time, as in the reference.
 """
 from __future__ import annotations
 
@@ -30,6 +40,19 @@ from .io import (check_valid_options, default_options_initializer, get_input_fil
                  save_numpy_as_dat, write_rows)
 
 log = logging.getLogger(__name__)
+
+
+def _write_atomic(path, write):
+    """write(tmp) to a temporary file beside `path`, then rename it to
+    `path` (atomic on one filesystem)."""
+    tmp = f"{path}.part{os.getpid()}"
+    try:
+        write(tmp)
+        os.replace(tmp, path)
+    finally:
+        if os.path.exists(tmp):
+            os.unlink(tmp)
+
 
 HOT_METHODS = ("topo", "topo_GPU", "volume", "volume_ESP", "point_field", "point_mag")
 
@@ -90,6 +113,13 @@ class CPET:
     def _topo_group(self, group, runtype):
         """Streamline features of the frames of `group` ((frame, calculator)
         pairs), one batched launch for several, then their .top files."""
+        done = set(os.listdir(self.outputpath))
+        for frame, _ in group:
+            if frame + ".top" in done:
+                self.log.info("Finished by another process meanwhile: %s, skipping...", frame)
+        group = [(frame, c) for frame, c in group if frame + ".top" not in done]
+        if not group:
+            return
         euler = all(getattr(c, "integrator", "euler") == "euler" for _, c in group)
         batched = len(group) > 1 and euler  # the batch kernel integrates with Euler
         if batched:
@@ -104,7 +134,8 @@ class CPET:
         for frame, c in group:
             self.calculator = c
             hist = c.hist if batched else getattr(c, runtype)()
-            write_rows(self.outputpath + f"/{frame}.top", hist, "%.18e")
+            _write_atomic(self.outputpath + f"/{frame}.top",
+                          lambda path: write_rows(path, hist, "%.18e"))
 
     def _meta(self, mesh_shape):
         c = self.calculator
@@ -116,15 +147,19 @@ class CPET:
         for f, frame in self._frames("_efield.dat"):
             self.calculator = calculator(self.options, path_to_pdb=f)
             field_box, mesh_shape = self.calculator.compute_box()
-            save_numpy_as_dat(name=self.outputpath + f"/{frame}_efield.dat", volume=field_box,
-                              meta_data=self._meta(mesh_shape))
+            meta = self._meta(mesh_shape)
+            _write_atomic(self.outputpath + f"/{frame}_efield.dat",
+                          lambda path: save_numpy_as_dat(name=path, volume=field_box,
+                                                         meta_data=meta))
 
     def run_volume_ESP(self):
         for f, frame in self._frames("_esp.dat"):
             self.calculator = calculator(self.options, path_to_pdb=f)
             esp_box, mesh_shape = self.calculator.compute_box_ESP()
-            save_numpy_as_dat(name=self.outputpath + f"/{frame}_esp.dat", volume=esp_box,
-                              meta_data=self._meta(mesh_shape))
+            meta = self._meta(mesh_shape)
+            _write_atomic(self.outputpath + f"/{frame}_esp.dat",
+                          lambda path: save_numpy_as_dat(name=path, volume=esp_box,
+                                                         meta_data=meta))
 
     def run_point(self):
         files = get_input_files(self.inputpath, filetype=self.inputfiletype)

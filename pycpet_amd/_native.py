@@ -95,6 +95,7 @@ SIGNATURES = {
     "cpet_read_rows": [ctypes.c_char_p, c_int, P, ctypes.c_int64, P],
     "cpet_pdb_scan": [ctypes.c_char_p, P],
     "cpet_pdb_parse": [ctypes.c_char_p, c_int, c_int, P, P, P, P, P, P, P],
+    "cpet_pqr_parse": [ctypes.c_char_p, c_int, c_int, P, P, P, P, P, P],
     "cpet_write_rows": [ctypes.c_char_p, ctypes.c_char_p, P, c_int, ctypes.c_int64, c_int, c_int,
                         c_int],
     "cpet_batch_volume": [c_void_p, c_int, P, P, P, P, P, P, P, c_int, c_uint32],

@@ -134,6 +134,7 @@ enum Slot : int {
   kSlotCluster1,     // cluster step: edges, offsets, counts
   kSlotCluster2,     // cluster step: output
   kSlotNear,         // grid kernels: charge-near-grid flags
+  kSlotFar,          // streamlines: far-field charge records + per-frame flags
   kSlotCount
 };
 

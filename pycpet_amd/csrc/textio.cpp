@@ -269,6 +269,89 @@ int cpet_pdb_parse(const char* path, int n_atoms, int get_charges, double* xyz, 
   return rc;
 }
 
+// PQR records as parse_pqr reads them (CPET/utils/io.py:478-571): tokens of
+// line.split(); a first token longer than 6 characters ("HETATM12345") holds
+// the serial from character 6 and shifts the other tokens left by one; the
+// residue number is token 5 (4 when shifted), or, when that token is longer
+// than 4 characters, token 4 (3) without its first character; residue
+// numbers above 999 move the fixed columns right by digits - 3 (the
+// reference's int(log10(n) - 2)); x/y/z are columns [30:38] [38:46] [46:54]
+// and the charge [54:61], shifted.  Token fields are copied NUL-padded into
+// `width`-byte slots.  Any record Python would reject, or that this parser
+// does not read the way Python would (non-ASCII bytes, a token longer than
+// width - 1, digit-group underscores), returns CPET_ERR_ARG naming the line,
+// so the caller can fall back to the line-by-line restatement.
+int cpet_pqr_parse(const char* path, int n_atoms, int width, double* xyz, double* q,
+                   int32_t* atom_number, int32_t* resnum, char* atom_type, char* resname) {
+  if (!path || n_atoms < 0 || width < 2 || !xyz || !q || !atom_number || !resnum ||
+      !atom_type || !resname) {
+    set_error("cpet_pqr_parse: bad arguments");
+    return CPET_ERR_ARG;
+  }
+  std::string s;
+  if (!read_file(path, s)) {
+    set_error("cannot read %s", path);
+    return CPET_ERR_ARG;
+  }
+  int i = 0, lineno = 0, rc = CPET_OK;
+  for_lines(s, [&](const char* l, size_t n) {
+    ++lineno;
+    if (rc != CPET_OK || !is_record(l, n)) return;
+    auto fail = [&](const char* why) {
+      set_error("%s line %d: %s", path, lineno, why);
+      rc = CPET_ERR_ARG;
+    };
+    if (!is_ascii(l, n)) return fail("non-ASCII bytes");
+    if (i >= n_atoms) return fail("more atom records than counted");
+    Field tok[8];
+    int nt = 0;
+    for (size_t k = 0; k < n && nt < 8;) {  // the first tokens of line.split()
+      while (k < n && py_space((unsigned char)l[k])) ++k;
+      size_t e = k;
+      while (e < n && !py_space((unsigned char)l[e])) ++e;
+      if (e > k) tok[nt++] = Field{l + k, e - k};
+      k = e;
+    }
+    const bool merged = nt > 0 && tok[0].n > 6;
+    const int res_ind = merged ? 4 : 5;
+    if (nt <= res_ind) return fail("too few fields");
+    const Field serial = merged ? Field{tok[0].p + 6, tok[0].n - 6} : tok[1];
+    const Field aname = merged ? tok[1] : tok[2];
+    const Field rname = merged ? tok[2] : tok[3];
+    const Field rtok = tok[res_ind].n > 4 ? Field{tok[res_ind - 1].p + 1, tok[res_ind - 1].n - 1}
+                                          : tok[res_ind];
+    int32_t rv = 0;
+    if (!parse_int(serial, &atom_number[i]) || !parse_int(rtok, &rv))
+      return fail("invalid serial or residue number");
+    if (aname.n >= (size_t)width || rname.n >= (size_t)width) return fail("token too long");
+    int shift = 0;
+    if (rv > 999) {
+      int digits = 0;
+      for (int64_t t = rv; t > 0; t /= 10) ++digits;
+      shift = digits - 3;
+    }
+    double v[4];
+    if (!parse_float(field(l, n, 30 + shift, 38 + shift), &v[0]) ||
+        !parse_float(field(l, n, 38 + shift, 46 + shift), &v[1]) ||
+        !parse_float(field(l, n, 46 + shift, 54 + shift), &v[2]) ||
+        !parse_float(field(l, n, 54 + shift, 61 + shift), &v[3]))
+      return fail("charge or coordinates are not usable numbers");
+    xyz[3 * i] = v[0];
+    xyz[3 * i + 1] = v[1];
+    xyz[3 * i + 2] = v[2];
+    q[i] = v[3];
+    resnum[i] = rv;
+    copy_fixed(atom_type + (size_t)width * i, (size_t)width, aname);
+    copy_fixed(resname + (size_t)width * i, (size_t)width, rname);
+    ++i;
+  });
+  if (rc == CPET_OK && i != n_atoms) {
+    set_error("%s has %d atom records, expected %d", path, i, n_atoms);
+    rc = CPET_ERR_ARG;
+  }
+  return rc;
+}
+
 int cpet_write_rows(const char* path, const char* header, const void* data, int dtype,
                     int64_t rows, int cols, int fmt, int n_threads) {
   if (!path || !data || rows < 0 || cols <= 0 || dtype < 0 || dtype > 2 || fmt < 0 || fmt > 1) {

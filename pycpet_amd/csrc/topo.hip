@@ -34,6 +34,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "cpet_internal.h"
 
@@ -219,11 +220,14 @@ struct PropArgs {
   int32_t* steps;   // (L)
   int32_t* exited;  // (L)
   unsigned int* counter;
+  const float4* far;       // far-field records (2 per padded charge), or null
+  const int* far_unsafe;   // 0: every charge is far from the walk region
 };
 
 // Streams source records [r0, r1) (float4; r0 and r1 multiples of four) to
-// body(c, fx, fy, fz), four records at a time, summing FP32 over chunks of
-// kRec records from r0 and FP64 across chunks.  The records pass through two
+// body(c, f), four records at a time, body accumulating NC FP32 sums f[NC];
+// the sums are FP32 over chunks of kRec records from r0 and FP64 across
+// chunks (into e[NC]).  The records pass through two
 // SGPR buffers of four float4 (one 64-byte scalar load each).  The load of
 // the next group is issued only after the current group has arrived (an
 // explicit lgkmcnt(0) wait: scalar loads return out of order, so any later
@@ -232,13 +236,14 @@ struct PropArgs {
 // stream runs on across chunk boundaries; the last prefetch is clamped to
 // the range's last group, so no load leaves the range.  A last partial chunk
 // runs whole group pairs and then at most one single group.
-template <int kRec, class Body>
+template <int kRec, int NC, class Body>
 __device__ __forceinline__ void stream_recs(const float4* __restrict__ ch, int r0, int r1,
-                                            Body body, double& ex, double& ey, double& ez) {
+                                            Body body, double (&e)[NC]) {
   constexpr int kGroup = 4;
   constexpr int kWaitLgkm0 = 0xc07f;  // s_waitcnt lgkmcnt(0), other counters untouched
   static_assert(kRec % (2 * kGroup) == 0, "a chunk holds whole group pairs");
-  ex = ey = ez = 0.0;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) e[c] = 0.0;
   if (r0 >= r1) return;
   const int last = r1 - kGroup;  // index of the range's last group
   int nx = r0;
@@ -247,14 +252,14 @@ __device__ __forceinline__ void stream_recs(const float4* __restrict__ ch, int r
   for (int u = 0; u < kGroup; ++u) a[u] = ch[nx + u];
   // One group pair: a has been issued; b (the next group, which exists) is
   // loaded during a's arithmetic and the group after it into a during b's.
-  auto pair = [&](float& fx, float& fy, float& fz) __attribute__((always_inline)) {
+  auto pair = [&](float (&f)[NC]) __attribute__((always_inline)) {
     __builtin_amdgcn_sched_barrier(0);
     __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
     nx += kGroup;
 #pragma unroll
     for (int u = 0; u < kGroup; ++u) b[u] = ch[nx + u];
     __builtin_amdgcn_sched_barrier(0);
-    body(a, fx, fy, fz);
+    body(a, f);
     __builtin_amdgcn_sched_barrier(0);
     __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
     nx += kGroup;
@@ -262,29 +267,31 @@ __device__ __forceinline__ void stream_recs(const float4* __restrict__ ch, int r
 #pragma unroll
     for (int u = 0; u < kGroup; ++u) a[u] = ch[pf + u];
     __builtin_amdgcn_sched_barrier(0);
-    body(b, fx, fy, fz);
+    body(b, f);
   };
   int r = r0;
   for (; r + kRec <= r1; r += kRec) {
-    float fx = 0.f, fy = 0.f, fz = 0.f;
+    float f[NC];
 #pragma unroll
-    for (int g = 0; g < kRec; g += 2 * kGroup) pair(fx, fy, fz);
-    ex += (double)fx;
-    ey += (double)fy;
-    ez += (double)fz;
+    for (int c = 0; c < NC; ++c) f[c] = 0.f;
+#pragma unroll
+    for (int g = 0; g < kRec; g += 2 * kGroup) pair(f);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) e[c] += (double)f[c];
   }
   if (r < r1) {
-    float fx = 0.f, fy = 0.f, fz = 0.f;
-    for (; r + 2 * kGroup <= r1; r += 2 * kGroup) pair(fx, fy, fz);
+    float f[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) f[c] = 0.f;
+    for (; r + 2 * kGroup <= r1; r += 2 * kGroup) pair(f);
     if (r < r1) {  // one group left, already loaded into a
       __builtin_amdgcn_sched_barrier(0);
       __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
       __builtin_amdgcn_sched_barrier(0);
-      body(a, fx, fy, fz);
+      body(a, f);
     }
-    ex += (double)fx;
-    ey += (double)fy;
-    ez += (double)fz;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) e[c] += (double)f[c];
   }
 }
 
@@ -331,22 +338,23 @@ __device__ __forceinline__ void wave_records(int rec0, int nrec, int wave, int S
 // Charges: one float4 (x, y, z, q) per source (math_module.c:296-333).
 struct ChargeSrc {
   static constexpr int kRecPerSrc = 1;
+  static constexpr int kComp = 3;  // the slice sums are the field components
   // Unsoftened field of charge records [r0, r1) at one point (the k factor
   // is applied by the caller).
   __device__ static void slice(float px, float py, float pz, const float4* __restrict__ ch,
-                               int r0, int r1, double& ex, double& ey, double& ez) {
-    stream_recs<kTile>(ch, r0, r1, [&](const float4 (&c)[4], float& fx, float& fy, float& fz) {
+                               int r0, int r1, double (&e)[3]) {
+    stream_recs<kTile, 3>(ch, r0, r1, [&](const float4 (&c)[4], float (&f)[3]) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const float dx = px - c[u].x, dy = py - c[u].y, dz = pz - c[u].z;
         const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
         const float ri = __builtin_amdgcn_rsqf(r2);
         const float s = (c[u].w * ri) * (ri * ri);
-        fx = fmaf(s, dx, fx);
-        fy = fmaf(s, dy, fy);
-        fz = fmaf(s, dz, fz);
+        f[0] = fmaf(s, dx, f[0]);
+        f[1] = fmaf(s, dy, f[1]);
+        f[2] = fmaf(s, dz, f[2]);
       }
-    }, ex, ey, ez);
+    }, e);
   }
   // Sources [g0, g1), both multiples of 16.
   __device__ static void field2_f64(const double (&p)[2][3], double (&e)[2][3],
@@ -383,12 +391,13 @@ struct ChargeSrc {
 template <bool PHYS>
 struct DipoleSrc {
   static constexpr int kRecPerSrc = 2;
+  static constexpr int kComp = 3;
   __device__ static float coef(float mx, float my, float mz, float dx, float dy, float dz) {
     return PHYS ? 3.f * fmaf(mz, dz, fmaf(my, dy, mx * dx)) : fmaf(3.f * mx, dx, fmaf(my, dy, mz * dz));
   }
   __device__ static void slice(float px, float py, float pz, const float4* __restrict__ ch,
-                               int r0, int r1, double& ex, double& ey, double& ez) {
-    stream_recs<2 * kTile>(ch, r0, r1, [&](const float4 (&c)[4], float& fx, float& fy, float& fz) {
+                               int r0, int r1, double (&e)[3]) {
+    stream_recs<2 * kTile, 3>(ch, r0, r1, [&](const float4 (&c)[4], float (&f)[3]) {
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
         const float4 X = c[2 * u], M = c[2 * u + 1];
@@ -398,11 +407,11 @@ struct DipoleSrc {
         const float ri2 = ri * ri;
         const float ri5 = (ri2 * ri2) * ri;
         const float cc = coef(M.x, M.y, M.z, dx, dy, dz);
-        fx = fmaf(ri5, fmaf(cc, dx, -M.x * r2), fx);
-        fy = fmaf(ri5, fmaf(cc, dy, -M.y * r2), fy);
-        fz = fmaf(ri5, fmaf(cc, dz, -M.z * r2), fz);
+        f[0] = fmaf(ri5, fmaf(cc, dx, -M.x * r2), f[0]);
+        f[1] = fmaf(ri5, fmaf(cc, dy, -M.y * r2), f[1]);
+        f[2] = fmaf(ri5, fmaf(cc, dz, -M.z * r2), f[2]);
       }
-    }, ex, ey, ez);
+    }, e);
   }
   // Sources [g0, g1), both multiples of 16.
   __device__ static void field2_f64(const double (&p)[2][3], double (&e)[2][3],
@@ -455,9 +464,9 @@ struct IsRk4<Rk4<Base>> {
 // the GPU, not a fast path.  The epilogue is the charge policy's (FP64).
 struct ChargeSrcF64 : ChargeSrc {
   __device__ static void slice64(double px, double py, double pz, const float4* __restrict__ ch,
-                                 int r0, int r1, double& ex, double& ey, double& ez) {
+                                 int r0, int r1, double (&e)[3]) {
     constexpr int kGroup = 4;
-    ex = ey = ez = 0.0;
+    double ex = 0.0, ey = 0.0, ez = 0.0;
     for (int j = r0; j < r1; j += kGroup) {
       float4 c[kGroup];
 #pragma unroll
@@ -474,6 +483,9 @@ struct ChargeSrcF64 : ChargeSrc {
         ez = fma(s, dz, ez);
       }
     }
+    e[0] = ex;
+    e[1] = ey;
+    e[2] = ez;
   }
 };
 template <class Src>
@@ -486,6 +498,97 @@ struct IsF64<ChargeSrcF64> {
 };
 template <class Base>
 struct IsF64<Rk4<Base>> : IsF64<Base> {};
+
+// Charges far from the walk region, in the expanded form of the pair term:
+//   r^2 = |p|^2 + |c|^2 - 2 p.c,   w = r^-3,
+//   S0 = sum q w,   S = sum (q c) w,   E / k = p S0 - S,
+// four FP32 sums per tile (FP64 across tiles).  Per pair that is 3 FMA + 1
+// add for r^2, the rsq, 2 multiplies for w and 4 FMA: 11 VALU operations
+// against the direct form's 13 (3 subtractions, 3 for r^2, rsq, 3 for
+// q r^-3, 3 FMA).  Two float4 records per charge: (x, y, z, |c|^2) and
+// (q, q x, q y, q z), built by far_pack_kernel.
+//
+// The expansion loses accuracy when |p| or |c| is large next to r = |p - c|
+// (the absolute rounding of r^2 is ~ u (|p| + |c|)^2).  far_pack_kernel
+// therefore checks, per frame, that every charge has
+//   |dims + h| + |c| <= kFarRatio * dist(c, box grown by h),
+// the region every walk point and curvature step stays in; otherwise the
+// frame runs the direct form.  Then the relative rounding of each r^2 stays
+// below ~ kFarRatio^2 * 3 u, and on the 5,000-charge / +-1.5 A streamline
+// workload (charges at least 3 A from the box) the field errors of the two
+// forms against FP64 are the same size (max 4.9e-7 and 4.4e-7 of |E| in a
+// float32 emulation over 60 points).
+constexpr float kFarRatio = 3.0f;
+
+struct ChargeSrcFar {
+  static constexpr int kRecPerSrc = 2;
+  static constexpr int kComp = 4;  // S0, Sx, Sy, Sz
+  __device__ static void slice(float px, float py, float pz, const float4* __restrict__ ch,
+                               int r0, int r1, double (&e)[4]) {
+    // -2 p (exact) and |p|^2, once per evaluation
+    const float mx = -2.f * px, my = -2.f * py, mz = -2.f * pz;
+    const float pp = fmaf(pz, pz, fmaf(py, py, px * px));
+    stream_recs<2 * kTile, 4>(ch, r0, r1, [&](const float4 (&c)[4], float (&f)[4]) {
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const float4 X = c[2 * u], Q = c[2 * u + 1];
+        float r2 = fmaf(mz, X.z, pp);
+        r2 = fmaf(my, X.y, r2);
+        r2 = fmaf(mx, X.x, r2);
+        r2 = r2 + X.w;
+        const float t = __builtin_amdgcn_rsqf(r2);
+        const float w = (t * t) * t;
+        f[0] = fmaf(Q.x, w, f[0]);
+        f[1] = fmaf(Q.y, w, f[1]);
+        f[2] = fmaf(Q.z, w, f[2]);
+        f[3] = fmaf(Q.w, w, f[3]);
+      }
+    }, e);
+  }
+};
+
+// Far-field records of padded charges [0, total) of the frame(s), and the
+// per-frame safety flag (unsafe[f] = 1 when a charge breaks the distance
+// condition above; cleared by the caller).  hd (F,4) = h, d0, d1, d2 and
+// chg_off (F+1) the padded charge offsets of a batch; for one frame (F == 1)
+// both are null and h1 = (h, d0, d1, d2).
+// Padding charges (q = 0 at 1e18) give (1e18, 1e18, 1e18, 3e36) and zeros:
+// their terms are exactly 0 and they are always far.  Every writer of a
+// flag stores the same 1, so no atomics are needed.
+__global__ __launch_bounds__(256) void far_pack_kernel(int total, const float4* __restrict__ ch,
+                                                       int F, const int32_t* __restrict__ chg_off,
+                                                       const double* __restrict__ hd, double4 h1,
+                                                       float4* __restrict__ far,
+                                                       int* __restrict__ unsafe) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= total) return;
+  int fr = 0;
+  if (F > 1) {
+    int lo = 0, hi = F - 1;  // the last frame f with chg_off[f] <= j
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (chg_off[mid] <= j)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    fr = lo;
+  }
+  const float4 c = ch[j];
+  const float cc = fmaf(c.z, c.z, fmaf(c.y, c.y, c.x * c.x));
+  far[2 * j] = make_float4(c.x, c.y, c.z, cc);
+  far[2 * j + 1] = make_float4(c.w, c.w * c.x, c.w * c.y, c.w * c.z);
+  // distance from c to the box grown by h, and the walk region's radius
+  const double4 fh = hd ? make_double4(hd[4 * fr], hd[4 * fr + 1], hd[4 * fr + 2], hd[4 * fr + 3])
+                       : h1;
+  const double g0 = fh.y + fh.x, g1 = fh.z + fh.x, g2 = fh.w + fh.x;
+  const double o0 = fmax(fabs((double)c.x) - g0, 0.0), o1 = fmax(fabs((double)c.y) - g1, 0.0),
+               o2 = fmax(fabs((double)c.z) - g2, 0.0);
+  const double dist = sqrt(o0 * o0 + o1 * o1 + o2 * o2);
+  const double reach = sqrt(g0 * g0 + g1 * g1 + g2 * g2) +
+                       sqrt((double)c.x * c.x + (double)c.y * c.y + (double)c.z * c.z);
+  if (!(reach <= (double)kFarRatio * dist)) unsafe[fr] = 1;
+}
 
 enum SlotState : int { kActive = 0, kNeed = 1, kDone = 2 };
 
@@ -506,19 +609,26 @@ struct FrameView {
 
 // Runs the workgroup's 64 slots until the frame's queue is exhausted.  Slot
 // state is identical in every wave: each wave sums its charge slice for all
-// 64 slots, the partials are reduced in a fixed order through LDS, and all
-// waves apply the same FP64 update.  With S >= 4 waves, waves 0-2 reduce one
-// field component each into esum and the others wait at a second barrier:
-// one pass over the partials instead of one per wave, which would be S times
-// the LDS reads of the same sums.  A finished slot asks the queue for the
-// next streamline (one idle iteration).  first_q >= 0 gives the slot an
-// initial queue position, first_q < 0 starts it empty.
+// 64 slots (Src::kComp sums per slot), the partials are reduced in a fixed
+// order through LDS, and all waves apply the same FP64 update.  With S >= 4
+// waves, waves 0 .. kComp-1 reduce one sum each into esum and the others
+// wait at a second barrier: one pass over the partials instead of one per
+// wave, which would be S times the LDS reads of the same sums; the second
+// barrier also orders the next step's partial writes after every read, so
+// one partial buffer suffices (S < 4 alternates two).  A finished slot asks
+// the queue for the next streamline (one idle iteration).  first_q >= 0
+// gives the slot an initial queue position, first_q < 0 starts it empty.
+// part holds kPartBufs(S) * S * 4 * 64 doubles, esum 4 * 64.
+__host__ __device__ constexpr int kPartBufs(int S) { return S >= 4 ? 1 : 2; }
+
 template <int S, class Src>
 __device__ __forceinline__ void run_slots(const float4* __restrict__ ch, const FrameView& f,
-                                          int first_q, double (*part)[S][3][64],
-                                          double (*esum)[64], int (*nsid)[64], int lane,
-                                          int wave) {
+                                          int first_q, double* part, double* esum,
+                                          int (*nsid)[64], int lane, int wave) {
   constexpr bool rk4 = IsRk4<Src>::value;
+  constexpr int NC = Src::kComp;
+  constexpr int NB = kPartBufs(S);
+  static_assert(S >= 4 ? NC <= S : true, "one reducing wave per sum");
   int st = kDone, sid = 0, step = 0, nmax = 0;
   double px = 0.0, py = 0.0, pz = 0.0;
   // RK4: the stage (0-3), its evaluation point and the weighted direction sum
@@ -562,44 +672,58 @@ __device__ __forceinline__ void run_slots(const float4* __restrict__ ch, const F
     st = kNeed;
   int par = 0;
   while (__ballot(st != kDone) != 0ull) {
-    double fx = 0.0, fy = 0.0, fz = 0.0;
+    double fc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) fc[c] = 0.0;
+    // the evaluation point as the FP32 slice sees it
+    const float ex32 = (float)(rk4 ? qx : px), ey32 = (float)(rk4 ? qy : py),
+                ez32 = (float)(rk4 ? qz : pz);
     if (st == kActive) {
       if constexpr (IsF64<Src>::value)
-        Src::slice64(rk4 ? qx : px, rk4 ? qy : py, rk4 ? qz : pz, ch, f.r0, f.r1, fx, fy, fz);
+        Src::slice64(rk4 ? qx : px, rk4 ? qy : py, rk4 ? qz : pz, ch, f.r0, f.r1, fc);
       else
-        Src::slice((float)(rk4 ? qx : px), (float)(rk4 ? qy : py), (float)(rk4 ? qz : pz), ch,
-                   f.r0, f.r1, fx, fy, fz);
+        Src::slice(ex32, ey32, ez32, ch, f.r0, f.r1, fc);
     }
-    part[par][wave][0][lane] = fx;
-    part[par][wave][1][lane] = fy;
-    part[par][wave][2][lane] = fz;
+    const int b = NB == 1 ? 0 : par;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) part[((b * S + wave) * NC + c) * 64 + lane] = fc[c];
     if (wave == 0 && st == kNeed)
       nsid[par][lane] = (int)atomicAdd(f.counter, 1u);
     __syncthreads();
     if constexpr (S >= 4) {
       // esum is read after the second barrier and rewritten only after the
       // next step's first barrier, which every reader has passed by then.
-      if (wave < 3) {
+      if (wave < NC) {
         double e = 0.0;
 #pragma unroll
-        for (int s = 0; s < S; ++s) e += part[par][s][wave][lane];
-        esum[wave][lane] = e;
+        for (int s = 0; s < S; ++s) e += part[(s * NC + wave) * 64 + lane];
+        esum[wave * 64 + lane] = e;
       }
       __syncthreads();
     }
     if (st == kActive) {
-      double ex = 0.0, ey = 0.0, ez = 0.0;
+      double ev[NC];
       if constexpr (S >= 4) {
-        ex = esum[0][lane];
-        ey = esum[1][lane];
-        ez = esum[2][lane];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) ev[c] = esum[c * 64 + lane];
       } else {
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
-          ex += part[par][s][0][lane];
-          ey += part[par][s][1][lane];
-          ez += part[par][s][2][lane];
-        }
+        for (int c = 0; c < NC; ++c) ev[c] = 0.0;
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+          for (int c = 0; c < NC; ++c) ev[c] += part[((b * S + s) * NC + c) * 64 + lane];
+      }
+      double ex, ey, ez;
+      if constexpr (NC == 4) {
+        // E / k = p S0 - S at the point the slice used
+        ex = fma((double)ex32, ev[0], -ev[1]);
+        ey = fma((double)ey32, ev[0], -ev[2]);
+        ez = fma((double)ez32, ev[0], -ev[3]);
+      } else {
+        ex = ev[0];
+        ey = ev[1];
+        ez = ev[2];
       }
       ex *= kCoulomb;
       ey *= kCoulomb;
@@ -664,8 +788,8 @@ __device__ __forceinline__ void run_slots(const float4* __restrict__ ch, const F
 template <int S, class Src>
 __global__ __launch_bounds__(64 * S, 8) void topo_propagate_kernel(
     const float4* __restrict__ ch, PropArgs a) {
-  __shared__ double part[2][S][3][64];
-  __shared__ double esum[3][64];
+  __shared__ double part[kPartBufs(S) * S * 4 * 64];
+  __shared__ double esum[4 * 64];
   __shared__ int nsid[2][64];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -682,6 +806,15 @@ __global__ __launch_bounds__(64 * S, 8) void topo_propagate_kernel(
   f.steps = a.steps;
   f.exited = a.exited;
   f.counter = a.counter;
+  if constexpr (std::is_same<Src, ChargeSrc>::value && S >= 4) {
+    if (a.far && __builtin_amdgcn_readfirstlane(*a.far_unsafe) == 0) {
+      constexpr int rps = ChargeSrcFar::kRecPerSrc;
+      const int nrec = used_records(a.far, 0, a.npad * rps, rps, lane);
+      wave_records(0, __builtin_amdgcn_readfirstlane(nrec), wave, S, f.r0, f.r1);
+      run_slots<S, ChargeSrcFar>(a.far, f, blockIdx.x * 64 + lane, part, esum, nsid, lane, wave);
+      return;
+    }
+  }
   const int nrec = used_records(ch, 0, a.npad * Src::kRecPerSrc, Src::kRecPerSrc, lane);
   wave_records(0, __builtin_amdgcn_readfirstlane(nrec), wave, S, f.r0, f.r1);
   run_slots<S, Src>(ch, f, blockIdx.x * 64 + lane, part, esum, nsid, lane, wave);
@@ -708,13 +841,15 @@ struct BatchTopoArgs {
   const int32_t* assign;      // frame id per assignment
   int n_assign;
   unsigned int* assign_counter;
+  const float4* far;          // far-field records at 2 x chg_off, or null
+  const int* far_unsafe;      // (F) 0: frame f's charges are far from its walk region
 };
 
 template <int S>
 __global__ __launch_bounds__(64 * S, 8) void topo_batch_kernel(const float4* __restrict__ ch,
                                                             BatchTopoArgs a) {
-  __shared__ double part[2][S][3][64];
-  __shared__ double esum[3][64];
+  __shared__ double part[kPartBufs(S) * S * 4 * 64];
+  __shared__ double esum[4 * 64];
   __shared__ int nsid[2][64];
   __shared__ int cur;
   const int lane = threadIdx.x & 63;
@@ -740,9 +875,17 @@ __global__ __launch_bounds__(64 * S, 8) void topo_batch_kernel(const float4* __r
     f.steps = a.steps + s0;
     f.exited = a.exited + s0;
     f.counter = a.sl_counter + fr;
-    const int nrec = used_records(ch, a.chg_off[fr], a.chg_off[fr + 1] - a.chg_off[fr], 1, lane);
-    wave_records(a.chg_off[fr], __builtin_amdgcn_readfirstlane(nrec), wave, S, f.r0, f.r1);
-    run_slots<S, ChargeSrc>(ch, f, -1, part, esum, nsid, lane, wave);
+    const int c0 = a.chg_off[fr], nc = a.chg_off[fr + 1] - c0;
+    if (S >= 4 && a.far && __builtin_amdgcn_readfirstlane(a.far_unsafe[fr]) == 0) {
+      constexpr int rps = ChargeSrcFar::kRecPerSrc;
+      const int nrec = used_records(a.far, rps * c0, rps * nc, rps, lane);
+      wave_records(rps * c0, __builtin_amdgcn_readfirstlane(nrec), wave, S, f.r0, f.r1);
+      run_slots<S, ChargeSrcFar>(a.far, f, -1, part, esum, nsid, lane, wave);
+    } else {
+      const int nrec = used_records(ch, c0, nc, 1, lane);
+      wave_records(c0, __builtin_amdgcn_readfirstlane(nrec), wave, S, f.r0, f.r1);
+      run_slots<S, ChargeSrc>(ch, f, -1, part, esum, nsid, lane, wave);
+    }
   }
 }
 
@@ -1054,6 +1197,25 @@ int topo_run(cpet_ctx* ctx, int N, int L, const float* starts,
     }
   }
 
+  // far-field records and their safety flag (plain charges, S >= 4)
+  int S = ctx->tune[2][0];
+  if (S <= 0) S = 16;  // 16 waves x 64 slots: fastest measured (S 4/8/16, G 256..2048)
+  const float4* d_far = nullptr;
+  const int* d_far_unsafe = nullptr;
+  if (src == kSrcCharge && S >= 4) {
+    void* fb = nullptr;
+    if ((rc = ctx_buf(ctx, kSlotFar, (size_t)npad * 2 * sizeof(float4) + 256, &fb))) return rc;
+    int* flag = (int*)((char*)fb + (size_t)npad * 2 * sizeof(float4));
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(256), 0, st, (unsigned int*)flag, 1LL, 0u);
+    hipLaunchKernelGGL(far_pack_kernel, dim3((npad + 255) / 256), dim3(256), 0, st, npad, ch, 1,
+                       (const int32_t*)nullptr, (const double*)nullptr,
+                       make_double4((double)h, (double)dims[0], (double)dims[1], (double)dims[2]),
+                       (float4*)fb, flag);
+    CPET_HIP_TRY(hipGetLastError());
+    d_far = (const float4*)fb;
+    d_far_unsafe = flag;
+  }
+
   // 1. longest-first order
   if ((rc = prof_mark(ctx, 0))) return rc;
   hipLaunchKernelGGL(fill_u32_kernel, dim3((kNumBuckets + 255) / 256), dim3(256), 0, st,
@@ -1066,9 +1228,7 @@ int topo_run(cpet_ctx* ctx, int N, int L, const float* starts,
   CPET_HIP_TRY(hipGetLastError());
 
   // 2. propagate
-  int S = ctx->tune[2][0];
   int G = ctx->tune[2][1];
-  if (S <= 0) S = 16;  // 16 waves x 64 slots: fastest measured (S 4/8/16, G 256..2048)
   const int gmax = (L + 63) / 64;
   // One 16-wave workgroup per CU: the fewest slots that keep every SIMD fed
   // (4 waves each, with the charge loads prefetched), so the tail of long
@@ -1092,6 +1252,8 @@ int topo_run(cpet_ctx* ctx, int N, int L, const float* starts,
   pa.steps = d_steps;
   pa.exited = d_exited;
   pa.counter = d_counter;
+  pa.far = d_far;
+  pa.far_unsafe = d_far_unsafe;
   if (S != 1 && S != 2 && S != 4 && S != 8 && S != 16) {
     set_error("unsupported waves per workgroup %d (1,2,4,8,16)", S);
     return CPET_ERR_ARG;
@@ -1289,6 +1451,26 @@ int topo_batch_run(cpet_ctx* ctx, int F, const int32_t* n_starts, const float* s
                      F, (const int32_t*)(tw + o_sl), (const int32_t*)diter, d_bk, d_ord);
   CPET_HIP_TRY(hipGetLastError());
 
+  // far-field records of every frame and the per-frame safety flags
+  const float4* d_far = nullptr;
+  const int* d_far_unsafe = nullptr;
+  if (S >= 4) {
+    const int total = chg_off[F];
+    void* fb = nullptr;
+    if ((rc = ctx_buf(ctx, kSlotFar, (size_t)total * 2 * sizeof(float4) + (size_t)F * 4 + 256,
+                      &fb)))
+      return rc;
+    int* flags_d = (int*)((char*)fb + (size_t)total * 2 * sizeof(float4));
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((F + 255) / 256), dim3(256), 0, st,
+                       (unsigned int*)flags_d, (long long)F, 0u);
+    hipLaunchKernelGGL(far_pack_kernel, dim3((total + 255) / 256), dim3(256), 0, st, total, ch, F,
+                       d_chg_off, (const double*)(tw + o_hd), make_double4(0.0, 0.0, 0.0, 0.0),
+                       (float4*)fb, flags_d);
+    CPET_HIP_TRY(hipGetLastError());
+    d_far = (const float4*)fb;
+    d_far_unsafe = flags_d;
+  }
+
   const int G = std::max(1, std::min<int>(G0, (int)assign.size()));
   BatchTopoArgs ba;
   ba.F = F;
@@ -1305,6 +1487,8 @@ int topo_batch_run(cpet_ctx* ctx, int F, const int32_t* n_starts, const float* s
   ba.assign = (const int32_t*)(tw + o_as);
   ba.n_assign = (int)assign.size();
   ba.assign_counter = d_cnt + F;
+  ba.far = d_far;
+  ba.far_unsafe = d_far_unsafe;
   if ((rc = prof_mark(ctx, 0))) return rc;
   switch (S) {
     case 1: hipLaunchKernelGGL((topo_batch_kernel<1>), dim3(G), dim3(64), 0, st, ch, ba); break;

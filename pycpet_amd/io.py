@@ -15,9 +15,10 @@ The radius and box filter decisions can also run on the GPU
 (pycpet_amd.frame); the functions here keep the reference's numpy
 semantics so both can be compared on identical coordinates.
 
-PDB parsing and the .dat / .top writers run in the native library
-(cpet_pdb_parse, cpet_write_rows: host code, no GPU needed), which
-reproduces the reference's parsed values and output bytes exactly.
+PDB and PQR parsing and the .dat / .top writers run in the native library
+(cpet_pdb_parse, cpet_pqr_parse, cpet_write_rows: host code, no GPU
+needed), which reproduces the reference's parsed values and output bytes
+exactly.
 """
 from __future__ import annotations
 
@@ -175,9 +176,48 @@ def filter_in_box(x, Q, ID, center, dimensions, keep=None):
 
 # ------------------------------------------------------------------ parsing
 
+_PQR_WIDTH = 16  # bytes per atom / residue name slot of the native parser
+
+
 def parse_pqr(path_to_pqr):
-    """PQR parser with the reference's column-shift rule for residue numbers
-    above 999 and its token logic for long record names."""
+    """PQR parser (CPET/utils/io.py:478-571): returns (x (N,3), Q (N,1),
+    atom_number, residue_name, residue_number, atom_type) with the
+    reference's dtypes, its token logic for merged record names and its
+    column shift for residue numbers above 999.  Parsing runs in
+    cpet_pqr_parse; a record that parser does not read exactly as Python
+    would (non-ASCII bytes, underscores in numbers, very long tokens) or that
+    is malformed is re-read by the line-by-line restatement, which returns
+    Python's result or raises Python's exception."""
+    import ctypes
+
+    from . import _native as nat
+
+    lib = nat.load()
+    path = str(path_to_pqr).encode()
+    n = ctypes.c_int(0)
+    if lib.cpet_pdb_scan(path, ctypes.byref(n)) != nat.CPET_OK:  # same record rule
+        raise OSError(nat.last_error(lib))
+    N = n.value
+    if N == 0:
+        return _parse_pqr_text(path_to_pqr)
+    W = _PQR_WIDTH
+    xyz = np.empty((N, 3), np.float64)
+    q = np.empty(N, np.float64)
+    anum = np.empty(N, np.int32)
+    rnum = np.empty(N, np.int32)
+    atype = np.zeros(W * N, np.uint8)
+    rname = np.zeros(W * N, np.uint8)
+    ptr = lambda a: a.ctypes.data
+    if lib.cpet_pqr_parse(path, N, W, ptr(xyz), ptr(q), ptr(anum), ptr(rnum), ptr(atype),
+                          ptr(rname)) != nat.CPET_OK:
+        return _parse_pqr_text(path_to_pqr)
+    return (xyz, q.reshape(-1, 1), anum.astype(np.int64), _fixed_str(rname, W),
+            rnum.astype(np.int64), _fixed_str(atype, W))
+
+
+def _parse_pqr_text(path_to_pqr):
+    """Line-by-line restatement of the reference PQR parser (the fallback of
+    parse_pqr, and its check)."""
     x, Q, atom_num, res_name, res_num, atom_type = [], [], [], [], [], []
     with open(path_to_pqr) as fh:
         lines = fh.readlines()

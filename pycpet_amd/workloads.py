@@ -172,6 +172,38 @@ def small_topo(N: int = 300, L: int = 256, cap: int = 400, seed: int = 5,
     return TopoCase(f"small_topo_{N}x{L}", x, q, starts, n_iter, h, d, ms)
 
 
+def md_base(n_max: int = 40000, seed: int = 3):
+    """The shared clustered charge set MD-like frames are cut from."""
+    return clustered(n_max // 10, 10, 45.0, 1.5, -3.0, seed)
+
+
+def md_frame(base, i: int, kind: str, n: int, *, edge: Optional[int] = None,
+             edges=(21, 41, 61, 81), seed: int = 3, topo_starts_per_frame: int = 20000,
+             h: float = 0.001, cap: int = 2000, topo_box: float = 1.5, jitter: float = 0.3,
+             rng=None):
+    """Frame i of the mixed MD-like set: the first n charges of `base` with
+    Gaussian position jitter, as a volume job on an edge^3 grid (step
+    0.1 A, half-width (edge-1)*0.05 A; edge drawn from `edges` after the
+    jitter when not given) or a streamline job with `topo_starts_per_frame`
+    starts in +-topo_box."""
+    base_x, base_q = base
+    if n > base_q.size:
+        raise ValueError(f"frame of {n} charges from a base of {base_q.size}")
+    rng = rng if rng is not None else np.random.default_rng(seed * 7919 + i)
+    x = (base_x[:n] + jitter * rng.standard_normal((n, 3))).astype(np.float32)
+    q = base_q[:n].copy()
+    if kind == "volume":
+        if edge is None:
+            edge = int(rng.choice(edges))
+        d = (edge - 1) * 0.05
+        ax = np.linspace(-d, d, edge).astype(np.float32)
+        return {"kind": "volume", "x": x, "q": q, "axes": (ax, ax, ax)}
+    dims = np.full(3, topo_box, np.float32)
+    st, it, _ = topo_starts(topo_starts_per_frame, dims, h, cap, seed=seed * 1000 + i,
+                            start_seed=seed * 1000 + i + 1)
+    return {"kind": "topo", "x": x, "q": q, "starts": st, "n_iter": it, "h": h, "dims": dims}
+
+
 def md_frames(n_frames: int = 512, seed: int = 3, n_min: int = 3000, n_max: int = 40000,
               edges=(21, 41, 61, 81), topo_starts_per_frame: int = 20000, h: float = 0.001,
               cap: int = 2000, topo_box: float = 1.5, jitter: float = 0.3):
@@ -181,21 +213,12 @@ def md_frames(n_frames: int = 512, seed: int = 3, n_min: int = 3000, n_max: int 
     so the box half-width is (edge-1)*0.05 A), odd frames are streamline
     jobs with `topo_starts_per_frame` starts in +-topo_box."""
     rng = np.random.default_rng(seed)
-    base_x, base_q = clustered(n_max // 10, 10, 45.0, 1.5, -3.0, seed)
+    base = md_base(n_max, seed)
     frames = []
     for i in range(n_frames):
         n = int(rng.integers(n_min, n_max + 1)) // 10 * 10
-        x = (base_x[:n] + jitter * rng.standard_normal((n, 3))).astype(np.float32)
-        q = base_q[:n].copy()
-        if i % 2 == 0:
-            e = int(rng.choice(edges))
-            d = (e - 1) * 0.05
-            ax = np.linspace(-d, d, e).astype(np.float32)
-            frames.append({"kind": "volume", "x": x, "q": q, "axes": (ax, ax, ax)})
-        else:
-            dims = np.full(3, topo_box, np.float32)
-            st, it, _ = topo_starts(topo_starts_per_frame, dims, h, cap, seed=seed * 1000 + i,
-                                    start_seed=seed * 1000 + i + 1)
-            frames.append({"kind": "topo", "x": x, "q": q, "starts": st, "n_iter": it, "h": h,
-                           "dims": dims})
+        kind = "volume" if i % 2 == 0 else "topo"
+        frames.append(md_frame(base, i, kind, n, edges=edges, seed=seed,
+                               topo_starts_per_frame=topo_starts_per_frame, h=h, cap=cap,
+                               topo_box=topo_box, jitter=jitter, rng=rng))
     return frames
