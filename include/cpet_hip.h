@@ -318,7 +318,7 @@ int cpet_ctx_phase_times(cpet_ctx *ctx, float *ms, int n);
 
 /* Number of grid / points the last field call on this context recomputed in
  * FP64 because the auto-precision check found strong cancellation there
- * (u * sum|q|/r^2 > 6e-6 * |E|); 0 when that call had no check (precision
+ * (u * sum|q|/r^2 > 4e-6 * |E|); 0 when that call had no check (precision
  * modes other than auto, or no E requested).  Waits for the context's
  * stream.  No counterpart in the reference (its sums are FP32 throughout,
  * CPET/utils/math_module.c:405-451). */

@@ -29,16 +29,41 @@ thread_local char g_err[512] = "";
 std::mutex g_default_mu;
 cpet_ctx* g_default = nullptr;
 
+// Packed charge j holds input charge (j * a) mod N, with a the first odd
+// number from 40503 up that is coprime to N: a fixed permutation that spreads
+// the input order (in which neighbouring rows are usually neighbouring atoms)
+// over the 64-charge tiles.  A tile is then a sample of the whole frame, so
+// its FP32 partial sums do not grow monotonically over a run of similar,
+// same-direction terms, whose roundings would be correlated.  Sums are
+// independent of the order only up to rounding; the permutation depends on
+// N alone, so a frame packs the same way in every entry point and batch.
+__device__ __forceinline__ long long shuffle_src(long long j, long long n) {
+  long long a = 40503;
+  for (;;) {  // gcd(a, n) == 1 ?
+    long long u = a, v = n;
+    while (v) {
+      const long long t = u % v;
+      u = v;
+      v = t;
+    }
+    if (u == 1) break;
+    a += 2;
+  }
+  return (j * a) % n;
+}
+
 __global__ __launch_bounds__(256) void pack_charges_kernel(
     int N, int npad, const float* __restrict__ x, const float* __restrict__ q,
     float4* __restrict__ out, double4* __restrict__ out64) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= npad) return;
   float4 c;
-  if (j < N)
-    c = make_float4(x[3 * j], x[3 * j + 1], x[3 * j + 2], q[j]);
-  else
+  if (j < N) {
+    const long long g = shuffle_src(j, N);
+    c = make_float4(x[3 * g], x[3 * g + 1], x[3 * g + 2], q[g]);
+  } else {
     c = make_float4(kPadCoord, kPadCoord, kPadCoord, 0.f);
+  }
   out[j] = c;
   if (out64) out64[j] = make_double4(c.x, c.y, c.z, c.w);
 }
@@ -78,7 +103,7 @@ __global__ __launch_bounds__(256) void pack_batch_kernel(
   const long long n = x_off[lo + 1] - x_off[lo];
   float4 c;
   if (loc < n) {
-    const long long g = x_off[lo] + loc;
+    const long long g = x_off[lo] + shuffle_src(loc, n);
     c = make_float4(x[3 * g], x[3 * g + 1], x[3 * g + 2], q[g]);
   } else {
     c = make_float4(kPadCoord, kPadCoord, kPadCoord, 0.f);
@@ -104,13 +129,40 @@ int ctx_activate(cpet_ctx* ctx) {
   return CPET_OK;
 }
 
+static_assert(kSlotCount <= 32, "arena slot table too small");
+
+void arena_reset(cpet_ctx* ctx) {
+  ctx->arena_used = 0;
+  for (int i = 0; i < 32; ++i) {
+    ctx->arena_slot[i] = nullptr;
+    ctx->arena_slot_bytes[i] = 0;
+  }
+}
+
 int ctx_buf(cpet_ctx* ctx, int slot, size_t bytes, void** out) {
   if (slot < 0 || slot >= kSlotCount) {
     set_error("internal: bad scratch slot %d", slot);
     return CPET_ERR_ARG;
   }
-  DevBuf& b = ctx->bufs[slot];
   if (bytes == 0) bytes = 256;
+  if (ctx->arena) {
+    if (ctx->arena_slot[slot] && ctx->arena_slot_bytes[slot] >= bytes) {
+      *out = ctx->arena_slot[slot];
+      return CPET_OK;
+    }
+    const size_t off = (ctx->arena_used + 255) & ~(size_t)255;
+    if (off > ctx->arena_bytes || bytes > ctx->arena_bytes - off) {
+      set_error("this call needs more than the %zu bytes of the fixed scratch array "
+                "(%zu in use, %zu more asked for)", ctx->arena_bytes, off, bytes);
+      return CPET_ERR_BUDGET;
+    }
+    ctx->arena_slot[slot] = ctx->arena + off;
+    ctx->arena_slot_bytes[slot] = bytes;
+    ctx->arena_used = off + bytes;
+    *out = ctx->arena + off;
+    return CPET_OK;
+  }
+  DevBuf& b = ctx->bufs[slot];
   if (b.bytes < bytes) {
     // tables cached in this buffer are lost with it
     for (auto& t : ctx->tables)
@@ -582,6 +634,27 @@ int cpet_topo(cpet_ctx* ctx, int N, int L, const float* starts,
     set_error("ctx is NULL");
     return CPET_ERR_ARG;
   }
+  // dims is a parameter like h: a host array in every mode, so a
+  // device-pointer call never waits on the stream (and can be captured)
+  float hdims[3];
+  int rc = topo_check_args(N, L, starts, n_iter, h, dims, x, q, feats, nb_d, edges_d, nb_c,
+                           edges_c, hist, hdims);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  rc = ctx_activate(ctx);
+  if (rc) return rc;
+  return topo_call(ctx, N, L, starts, n_iter, h, hdims, x, q, feats, steps, exited, nb_d,
+                   edges_d, nb_c, edges_c, hist, flags);
+}
+
+}  // extern "C"
+
+namespace cpet {
+
+int topo_check_args(int N, int L, const float* starts, const int32_t* n_iter, float h,
+                    const float* dims, const float* x, const float* q, const double* feats,
+                    int nb_d, const double* edges_d, int nb_c, const double* edges_c,
+                    const uint32_t* hist, float* hdims) {
   if (N <= 0 || L <= 0) {
     set_error("need N > 0 and L > 0 (got N=%d L=%d)", N, L);
     return CPET_ERR_ARG;
@@ -598,21 +671,22 @@ int cpet_topo(cpet_ctx* ctx, int N, int L, const float* starts,
     set_error("histogram requested without valid bins/edges");
     return CPET_ERR_ARG;
   }
-  const bool dev = (flags & CPET_F_DEVICE) != 0;
-  // dims is a parameter like h: a host array in every mode, so a
-  // device-pointer call never waits on the stream (and can be captured)
-  float hdims[3];
-  std::memcpy(hdims, dims, sizeof(hdims));
+  std::memcpy(hdims, dims, 3 * sizeof(float));
   if (!finite3(hdims) || !(hdims[0] > 0.f && hdims[1] > 0.f && hdims[2] > 0.f)) {
     set_error("box half-widths must be finite and > 0");
     return CPET_ERR_ARG;
   }
-  std::lock_guard<std::mutex> lk(ctx->mu);
-  int rc = ctx_activate(ctx);
-  if (rc) return rc;
+  return CPET_OK;
+}
+
+int topo_call(cpet_ctx* ctx, int N, int L, const float* starts, const int32_t* n_iter, float h,
+              const float* hdims, const float* x, const float* q, double* feats,
+              int32_t* steps, int32_t* exited, int nb_d, const double* edges_d, int nb_c,
+              const double* edges_c, uint32_t* hist, uint32_t flags) {
+  const bool dev = (flags & CPET_F_DEVICE) != 0;
   const float4* ch = nullptr;
   int npad = 0;
-  rc = pack_charges(ctx, N, x, q, dev, &ch, nullptr, &npad);
+  int rc = pack_charges(ctx, N, x, q, dev, &ch, nullptr, &npad);
   if (rc) return rc;
   return topo_run(ctx, N, L, starts, n_iter, h, hdims, ch, npad, feats, steps,
                   exited, nb_d, edges_d, nb_c, edges_c, hist, flags,
@@ -620,6 +694,10 @@ int cpet_topo(cpet_ctx* ctx, int N, int L, const float* starts,
                       ? ((flags & CPET_F_TOPO_RK4) ? kSrcChargeF64Rk4 : kSrcChargeF64)
                       : ((flags & CPET_F_TOPO_RK4) ? kSrcChargeRk4 : kSrcCharge));
 }
+
+}  // namespace cpet
+
+extern "C" {
 
 int cpet_topo_dipole(cpet_ctx* ctx, int N, int L, const float* starts, const int32_t* n_iter,
                      float h, const float* dims, const float* x, const float* mu,

@@ -102,6 +102,14 @@ struct cpet_ctx {
     std::vector<char> data;
   };
   TableCache tables[16];
+  // Fixed scratch arena (the allocation-free entry point): when set,
+  // ctx_buf carves every slot from it for the current call instead of
+  // allocating, and fails when it is too small.
+  char* arena = nullptr;
+  size_t arena_bytes = 0;
+  size_t arena_used = 0;
+  void* arena_slot[32] = {};
+  size_t arena_slot_bytes[32] = {};
 };
 
 namespace cpet {
@@ -138,8 +146,22 @@ enum Slot : int {
   kSlotCount
 };
 
-// Returns a device buffer of at least `bytes` for `slot`, growing it if needed.
+// Returns a device buffer of at least `bytes` for `slot`, growing it if needed
+// (or, for a context with an arena, a region of the arena).
 int ctx_buf(cpet_ctx* ctx, int slot, size_t bytes, void** out);
+// Forgets the arena regions of the previous call.
+void arena_reset(cpet_ctx* ctx);
+// cpet_topo after argument validation, with the context locked and active.
+int topo_call(cpet_ctx* ctx, int N, int L, const float* starts, const int32_t* n_iter, float h,
+              const float* hdims, const float* x, const float* q, double* feats,
+              int32_t* steps, int32_t* exited, int nb_d, const double* edges_d, int nb_c,
+              const double* edges_c, uint32_t* hist, uint32_t flags);
+// Validation shared by cpet_topo and its allocation-free variant; copies
+// dims into hdims.
+int topo_check_args(int N, int L, const float* starts, const int32_t* n_iter, float h,
+                    const float* dims, const float* x, const float* q, const double* feats,
+                    int nb_d, const double* edges_d, int nb_c, const double* edges_c,
+                    const uint32_t* hist, float* hdims);
 // Makes ctx->device current on the calling thread.
 int ctx_activate(cpet_ctx* ctx);
 // Copies host->device (or passes a device pointer through when dev != 0).

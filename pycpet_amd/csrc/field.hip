@@ -15,11 +15,22 @@
 //
 // Precision is chosen per quantity (flags bits 8..10):
 //   auto   E: FP32 pair terms, FP32 sum per 64-charge tile, FP64 across tiles,
-//          plus A = sum |q| / r^2 (the sum of the term magnitudes).  The FP32
-//          rounding error of E is at most ~0.6 u A (u = 2^-24; measured
-//          bound, the per-term roundings being independent), so a point with
-//          u A > kAutoTol |E| (strong cancellation) is appended to a list and
-//          recomputed in the "mixed" scheme by field_fix_kernel.
+//          plus A = sum |q| / r^2 (the sum of the term magnitudes).  A point
+//          with u A > kAutoTol |E| (u = 2^-24; strong cancellation) is
+//          appended to a list and recomputed in the "mixed" scheme by
+//          field_fix_kernel.  The worst-case FP32 error is about 140 u A
+//          (per term ~18 u through d, r^2, rsq and q r^-3, plus up to 63 u
+//          of the tile sum, per component), far above what occurs: the
+//          packed charges are shuffled (pack_charges), so a tile's terms are
+//          a sample of the frame, and the error measured on realistic and
+//          adversarial layouts (clustered frames, rock-salt lattices, shells,
+//          same-sign clusters of charges, up to 16 coincident copies, on
+//          both sides of a field zero) stays below ~2.5 u A at the accepted
+//          points, so kAutoTol = 4e-6 keeps them within 1e-5 |E| (emulated
+//          worst case 8.5e-6).  Many exactly coincident charges (64 copies
+//          of one charge) give every copy the same rounding, up to ~4 u A
+//          in all: merge coincident charges (sum their q) before the call,
+//          or use the "mixed" mode, when a frame has them.
 //          phi: "mixed" — FP64 displacement and r^2 (exact for FP32 inputs),
 //          an FP32 rsq seed refined by one FP64 Newton step, FP64 sums.
 //          (phi cancels strongly between charges of both signs, so FP32
@@ -75,9 +86,16 @@ __device__ __forceinline__ double rsqrt_refine(float seed, double x) {
   return fma(0.5 * y, e, y);
 }
 
+// The same step for x > 0 (no charge on the point): one clamp fewer.
+__device__ __forceinline__ double rsqrt_refine_pos(float seed, double x) {
+  const double y = (double)seed;
+  const double e = fma(-x, y * y, 1.0);
+  return fma(0.5 * y, e, y);
+}
+
 // Relative error target of the auto-mode check (see above): points whose
 // estimate u * A / |E| exceeds it are recomputed in FP64.
-constexpr double kAutoTol = 6e-6;
+constexpr double kAutoTol = 4e-6;
 constexpr double kUnit32 = 5.9604644775390625e-08;  // 2^-24
 
 // Slots of the per-point FP64 accumulator: E x, y, z, phi, A (auto E only).
@@ -93,8 +111,9 @@ __device__ __forceinline__ void kahan_add(float t, float& s, float& c) {
 // Accumulates the unscaled sums (sum q d / r^3, sum q / r and, for auto E,
 // sum |q| / r^2) of charges [t0*kTile, t1*kTile) at K points into
 // out[k][0..4] (FP64).  SHARED_XY: all K points share px[0], py[0] (grid
-// z-line).
-template <int K, bool SHARED_XY, int MODE, int WHAT>
+// z-line).  APART: no charge lies within 1e-3 A of a point (checked before
+// the launch), so r^2 > 0 and the FP64 refinement needs no clamp for 1/0.
+template <int K, bool SHARED_XY, int MODE, int WHAT, bool APART = false>
 __device__ __forceinline__ void accumulate(
     const float (&px)[K], const float (&py)[K], const float (&pz)[K],
     const float4* __restrict__ ch, const double4* __restrict__ chd, int t0,
@@ -210,7 +229,8 @@ __device__ __forceinline__ void accumulate(
           if constexpr (full64)
             y = rsqrt_f64(r2);
           else
-            y = rsqrt_refine(need32 ? ri : __builtin_amdgcn_rsqf((float)r2), r2);
+            y = APART ? rsqrt_refine_pos(need32 ? ri : __builtin_amdgcn_rsqf((float)r2), r2)
+                      : rsqrt_refine(need32 ? ri : __builtin_amdgcn_rsqf((float)r2), r2);
           if constexpr (p64) out[k][3] = fma(cd.w, y, out[k][3]);
           if constexpr (e64) {
             const double re = soft ? fmin(y, rclampd) : y;
@@ -401,6 +421,11 @@ __global__ __launch_bounds__(256) void field_grid_kernel(
   if constexpr (WHAT == kWhatESoft) {
     if (o.near && *o.near == 0)  // no charge near a grid point: the floor never binds
       accumulate<K, true, MODE, kWhatEHard>(px, py, pz, ch, chd, t0, t1, acc);
+    else
+      accumulate<K, true, MODE, WHAT>(px, py, pz, ch, chd, t0, t1, acc);
+  } else if constexpr (WHAT == kWhatPhi && !q_is32(qprec_p(MODE)) && MODE != kModeF64) {
+    if (o.near && *o.near == 0)  // r^2 >= 1e-6 at every pair: no 1/0 to keep
+      accumulate<K, true, MODE, WHAT, true>(px, py, pz, ch, chd, t0, t1, acc);
     else
       accumulate<K, true, MODE, WHAT>(px, py, pz, ch, chd, t0, t1, acc);
   } else {
@@ -847,10 +872,13 @@ int field_launch(cpet_ctx* ctx, bool grid_mode, int nx, int ny, int nz,
   int nzc = 0;
   const bool heavy = mode != kModeAuto && mode != kModeTiled && mode != kModeNaive;
   if (grid_mode) {
-    // K = 2 points per lane along z: measured fastest at every grid size
-    // (21^3..101^3, 5k..30k charges, against K = 1, 3, 4, 6, 8), where
-    // sharing dx, dy over more points saves less than the coarser waves cost.
-    K = forced_k > 0 ? forced_k : 2;
+    // K = 2 points per lane along z for E: measured fastest at every grid
+    // size (21^3..101^3, 5k..30k charges, against K = 1, 3, 4, 6, 8), where
+    // sharing dx, dy over more points saves less than the coarser waves
+    // cost.  The potential alone shares four FP64 operations per charge
+    // (dx, dy, dx^2 + dy^2) against ten per pair, and K = 4 is faster there
+    // (101^3 x 30k: 10.1 against 10.7 ms).
+    K = forced_k > 0 ? forced_k : (what == kWhatPhi ? 4 : 2);
     nzc = (nz + K - 1) / K;
     lanes = (long long)nx * ny * nzc;
   } else {
@@ -880,7 +908,7 @@ int field_launch(cpet_ctx* ctx, bool grid_mode, int nx, int ny, int nz,
     const int rc = fix_prepare(ctx, M, &o.fix_count, &o.fix_list);
     if (rc) return rc;
   }
-  if (grid_mode && what == kWhatESoft) {
+  if (grid_mode && (what == kWhatESoft || what == kWhatPhi)) {
     void* nf = nullptr;
     const int rc = ctx_buf(ctx, kSlotNear, sizeof(int), &nf);
     if (rc) return rc;

@@ -220,7 +220,6 @@ struct PropArgs {
   int32_t* steps;   // (L)
   int32_t* exited;  // (L)
   unsigned int* counter;
-  const float4* far;       // far-field records (2 per padded charge), or null
   const int* far_unsafe;   // 0: every charge is far from the walk region
 };
 
@@ -236,10 +235,9 @@ struct PropArgs {
 // stream runs on across chunk boundaries; the last prefetch is clamped to
 // the range's last group, so no load leaves the range.  A last partial chunk
 // runs whole group pairs and then at most one single group.
-template <int kRec, int NC, class Body>
+template <int kRec, int NC, int kGroup = 4, class Body>
 __device__ __forceinline__ void stream_recs(const float4* __restrict__ ch, int r0, int r1,
                                             Body body, double (&e)[NC]) {
-  constexpr int kGroup = 4;
   constexpr int kWaitLgkm0 = 0xc07f;  // s_waitcnt lgkmcnt(0), other counters untouched
   static_assert(kRec % (2 * kGroup) == 0, "a chunk holds whole group pairs");
 #pragma unroll
@@ -302,7 +300,7 @@ __device__ __forceinline__ void stream_recs(const float4* __restrict__ ch, int r
 // block of a multiple of kTile sources; lane l of the calling wave checks
 // source nsrc - 64 + l.  Every lane of the wave must call it.
 __device__ __forceinline__ int used_records(const float4* __restrict__ ch, int rec0, int nrec,
-                                            int rps, int lane) {
+                                            int rps, int lane, int gran = 4) {
   const int nsrc = nrec / rps;
   const int sidx = nsrc - 64 + lane;
   bool real = true;
@@ -319,17 +317,17 @@ __device__ __forceinline__ int used_records(const float4* __restrict__ ch, int r
   }
   const unsigned long long m = __ballot(real);
   const int used = m ? (nsrc - 64 + (63 - __clzll(m)) + 1) * rps : 0;
-  return min((used + 3) & ~3, nrec);
+  return min((used + gran - 1) / gran * gran, nrec);
 }
 
-// Splits n4 groups of four records (from record rec0) evenly over S waves:
-// wave w gets groups [w n4 / S, (w+1) n4 / S), so slices differ by at most
-// one group whatever the source count.
+// Splits nrec / gran groups of gran records (from record rec0) evenly over
+// S waves: wave w gets groups [w n / S, (w+1) n / S), so slices differ by at
+// most one group whatever the source count.
 __device__ __forceinline__ void wave_records(int rec0, int nrec, int wave, int S, int& r0,
-                                             int& r1) {
-  const long long n4 = nrec / 4;
-  r0 = rec0 + (int)(wave * n4 / S) * 4;
-  r1 = rec0 + (int)((wave + 1) * n4 / S) * 4;
+                                             int& r1, int gran = 4) {
+  const long long n = nrec / gran;
+  r0 = rec0 + (int)(wave * n / S) * gran;
+  r1 = rec0 + (int)((wave + 1) * n / S) * gran;
 }
 
 // Sources of the field a streamline follows.  A policy gives the FP32
@@ -522,15 +520,16 @@ constexpr float kFarRatio = 3.0f;
 
 struct ChargeSrcFar {
   static constexpr int kRecPerSrc = 2;
-  static constexpr int kComp = 4;  // S0, Sx, Sy, Sz
+  static constexpr int kComp = 4;   // S0, Sx, Sy, Sz
+  static constexpr int kGroup = 4;  // records per scalar-load group: 2 charges (8 overflow the SGPRs)
   __device__ static void slice(float px, float py, float pz, const float4* __restrict__ ch,
                                int r0, int r1, double (&e)[4]) {
     // -2 p (exact) and |p|^2, once per evaluation
     const float mx = -2.f * px, my = -2.f * py, mz = -2.f * pz;
     const float pp = fmaf(pz, pz, fmaf(py, py, px * px));
-    stream_recs<2 * kTile, 4>(ch, r0, r1, [&](const float4 (&c)[4], float (&f)[4]) {
+    stream_recs<2 * kTile, 4, kGroup>(ch, r0, r1, [&](const float4 (&c)[kGroup], float (&f)[4]) {
 #pragma unroll
-      for (int u = 0; u < 2; ++u) {
+      for (int u = 0; u < kGroup / 2; ++u) {
         const float4 X = c[2 * u], Q = c[2 * u + 1];
         float r2 = fmaf(mz, X.z, pp);
         r2 = fmaf(my, X.y, r2);
@@ -787,7 +786,7 @@ __device__ __forceinline__ void run_slots(const float4* __restrict__ ch, const F
 // kernel's stores.
 template <int S, class Src>
 __global__ __launch_bounds__(64 * S, 8) void topo_propagate_kernel(
-    const float4* __restrict__ ch, PropArgs a) {
+    const float4* __restrict__ ch, const float4* __restrict__ far, PropArgs a) {
   __shared__ double part[kPartBufs(S) * S * 4 * 64];
   __shared__ double esum[4 * 64];
   __shared__ int nsid[2][64];
@@ -807,11 +806,11 @@ __global__ __launch_bounds__(64 * S, 8) void topo_propagate_kernel(
   f.exited = a.exited;
   f.counter = a.counter;
   if constexpr (std::is_same<Src, ChargeSrc>::value && S >= 4) {
-    if (a.far && __builtin_amdgcn_readfirstlane(*a.far_unsafe) == 0) {
-      constexpr int rps = ChargeSrcFar::kRecPerSrc;
-      const int nrec = used_records(a.far, 0, a.npad * rps, rps, lane);
-      wave_records(0, __builtin_amdgcn_readfirstlane(nrec), wave, S, f.r0, f.r1);
-      run_slots<S, ChargeSrcFar>(a.far, f, blockIdx.x * 64 + lane, part, esum, nsid, lane, wave);
+    if (far && __builtin_amdgcn_readfirstlane(*a.far_unsafe) == 0) {
+      constexpr int rps = ChargeSrcFar::kRecPerSrc, gran = ChargeSrcFar::kGroup;
+      const int nrec = used_records(far, 0, a.npad * rps, rps, lane, gran);
+      wave_records(0, __builtin_amdgcn_readfirstlane(nrec), wave, S, f.r0, f.r1, gran);
+      run_slots<S, ChargeSrcFar>(far, f, blockIdx.x * 64 + lane, part, esum, nsid, lane, wave);
       return;
     }
   }
@@ -841,12 +840,12 @@ struct BatchTopoArgs {
   const int32_t* assign;      // frame id per assignment
   int n_assign;
   unsigned int* assign_counter;
-  const float4* far;          // far-field records at 2 x chg_off, or null
   const int* far_unsafe;      // (F) 0: frame f's charges are far from its walk region
 };
 
 template <int S>
 __global__ __launch_bounds__(64 * S, 8) void topo_batch_kernel(const float4* __restrict__ ch,
+                                                            const float4* __restrict__ far,
                                                             BatchTopoArgs a) {
   __shared__ double part[kPartBufs(S) * S * 4 * 64];
   __shared__ double esum[4 * 64];
@@ -876,11 +875,11 @@ __global__ __launch_bounds__(64 * S, 8) void topo_batch_kernel(const float4* __r
     f.exited = a.exited + s0;
     f.counter = a.sl_counter + fr;
     const int c0 = a.chg_off[fr], nc = a.chg_off[fr + 1] - c0;
-    if (S >= 4 && a.far && __builtin_amdgcn_readfirstlane(a.far_unsafe[fr]) == 0) {
-      constexpr int rps = ChargeSrcFar::kRecPerSrc;
-      const int nrec = used_records(a.far, rps * c0, rps * nc, rps, lane);
-      wave_records(rps * c0, __builtin_amdgcn_readfirstlane(nrec), wave, S, f.r0, f.r1);
-      run_slots<S, ChargeSrcFar>(a.far, f, -1, part, esum, nsid, lane, wave);
+    if (S >= 4 && far && __builtin_amdgcn_readfirstlane(a.far_unsafe[fr]) == 0) {
+      constexpr int rps = ChargeSrcFar::kRecPerSrc, gran = ChargeSrcFar::kGroup;
+      const int nrec = used_records(far, rps * c0, rps * nc, rps, lane, gran);
+      wave_records(rps * c0, __builtin_amdgcn_readfirstlane(nrec), wave, S, f.r0, f.r1, gran);
+      run_slots<S, ChargeSrcFar>(far, f, -1, part, esum, nsid, lane, wave);
     } else {
       const int nrec = used_records(ch, c0, nc, 1, lane);
       wave_records(c0, __builtin_amdgcn_readfirstlane(nrec), wave, S, f.r0, f.r1);
@@ -1123,19 +1122,21 @@ __global__ __launch_bounds__(kEpiWaves * kEpiLanes) void topo_batch_epilogue_ker
 }
 
 template <int S, class Src>
-hipError_t launch_prop(hipStream_t st, int G, const float4* ch, const PropArgs& a) {
-  hipLaunchKernelGGL((topo_propagate_kernel<S, Src>), dim3(G), dim3(64 * S), 0, st, ch, a);
+hipError_t launch_prop(hipStream_t st, int G, const float4* ch, const float4* far,
+                       const PropArgs& a) {
+  hipLaunchKernelGGL((topo_propagate_kernel<S, Src>), dim3(G), dim3(64 * S), 0, st, ch, far, a);
   return hipGetLastError();
 }
 
 template <class Src>
-hipError_t launch_prop_s(int S, hipStream_t st, int G, const float4* ch, const PropArgs& a) {
+hipError_t launch_prop_s(int S, hipStream_t st, int G, const float4* ch, const float4* far,
+                         const PropArgs& a) {
   switch (S) {
-    case 1: return launch_prop<1, Src>(st, G, ch, a);
-    case 2: return launch_prop<2, Src>(st, G, ch, a);
-    case 4: return launch_prop<4, Src>(st, G, ch, a);
-    case 8: return launch_prop<8, Src>(st, G, ch, a);
-    case 16: return launch_prop<16, Src>(st, G, ch, a);
+    case 1: return launch_prop<1, Src>(st, G, ch, far, a);
+    case 2: return launch_prop<2, Src>(st, G, ch, far, a);
+    case 4: return launch_prop<4, Src>(st, G, ch, far, a);
+    case 8: return launch_prop<8, Src>(st, G, ch, far, a);
+    case 16: return launch_prop<16, Src>(st, G, ch, far, a);
     default: return hipErrorInvalidValue;
   }
 }
@@ -1252,18 +1253,18 @@ int topo_run(cpet_ctx* ctx, int N, int L, const float* starts,
   pa.steps = d_steps;
   pa.exited = d_exited;
   pa.counter = d_counter;
-  pa.far = d_far;
   pa.far_unsafe = d_far_unsafe;
   if (S != 1 && S != 2 && S != 4 && S != 8 && S != 16) {
     set_error("unsupported waves per workgroup %d (1,2,4,8,16)", S);
     return CPET_ERR_ARG;
   }
-  hipError_t e = src == kSrcCharge       ? launch_prop_s<ChargeSrc>(S, st, G, ch, pa)
-                 : src == kSrcChargeRk4  ? launch_prop_s<Rk4<ChargeSrc>>(S, st, G, ch, pa)
-                 : src == kSrcChargeF64  ? launch_prop_s<ChargeSrcF64>(S, st, G, ch, pa)
-                 : src == kSrcChargeF64Rk4 ? launch_prop_s<Rk4<ChargeSrcF64>>(S, st, G, ch, pa)
-                 : src == kSrcDipoleRef  ? launch_prop_s<DipoleSrc<false>>(S, st, G, ch, pa)
-                                         : launch_prop_s<DipoleSrc<true>>(S, st, G, ch, pa);
+  hipError_t e =
+      src == kSrcCharge         ? launch_prop_s<ChargeSrc>(S, st, G, ch, d_far, pa)
+      : src == kSrcChargeRk4    ? launch_prop_s<Rk4<ChargeSrc>>(S, st, G, ch, nullptr, pa)
+      : src == kSrcChargeF64    ? launch_prop_s<ChargeSrcF64>(S, st, G, ch, nullptr, pa)
+      : src == kSrcChargeF64Rk4 ? launch_prop_s<Rk4<ChargeSrcF64>>(S, st, G, ch, nullptr, pa)
+      : src == kSrcDipoleRef    ? launch_prop_s<DipoleSrc<false>>(S, st, G, ch, nullptr, pa)
+                                : launch_prop_s<DipoleSrc<true>>(S, st, G, ch, nullptr, pa);
   if (e != hipSuccess) {
     set_error("topo propagate launch failed: %s", hipGetErrorString(e));
     return CPET_ERR_HIP;
@@ -1487,15 +1488,14 @@ int topo_batch_run(cpet_ctx* ctx, int F, const int32_t* n_starts, const float* s
   ba.assign = (const int32_t*)(tw + o_as);
   ba.n_assign = (int)assign.size();
   ba.assign_counter = d_cnt + F;
-  ba.far = d_far;
   ba.far_unsafe = d_far_unsafe;
   if ((rc = prof_mark(ctx, 0))) return rc;
   switch (S) {
-    case 1: hipLaunchKernelGGL((topo_batch_kernel<1>), dim3(G), dim3(64), 0, st, ch, ba); break;
-    case 2: hipLaunchKernelGGL((topo_batch_kernel<2>), dim3(G), dim3(128), 0, st, ch, ba); break;
-    case 4: hipLaunchKernelGGL((topo_batch_kernel<4>), dim3(G), dim3(256), 0, st, ch, ba); break;
-    case 8: hipLaunchKernelGGL((topo_batch_kernel<8>), dim3(G), dim3(512), 0, st, ch, ba); break;
-    case 16: hipLaunchKernelGGL((topo_batch_kernel<16>), dim3(G), dim3(1024), 0, st, ch, ba); break;
+    case 1: hipLaunchKernelGGL((topo_batch_kernel<1>), dim3(G), dim3(64), 0, st, ch, d_far, ba); break;
+    case 2: hipLaunchKernelGGL((topo_batch_kernel<2>), dim3(G), dim3(128), 0, st, ch, d_far, ba); break;
+    case 4: hipLaunchKernelGGL((topo_batch_kernel<4>), dim3(G), dim3(256), 0, st, ch, d_far, ba); break;
+    case 8: hipLaunchKernelGGL((topo_batch_kernel<8>), dim3(G), dim3(512), 0, st, ch, d_far, ba); break;
+    case 16: hipLaunchKernelGGL((topo_batch_kernel<16>), dim3(G), dim3(1024), 0, st, ch, d_far, ba); break;
     default:
       set_error("unsupported waves per workgroup %d (1,2,4,8,16)", S);
       return CPET_ERR_ARG;

@@ -291,3 +291,66 @@ def test_auto_recomputes_cancelling_points(gpu_ctx, oracle_mod):
             if grid else ops.field_points(pts, x, q, want_phi=True, out_f64=True, ctx=gpu_ctx)
         assert gpu_ctx.refined_points() == n_fixed
         np.testing.assert_array_equal(E2, E)
+
+
+def _adversarial(name):
+    """Charge layouts whose FP32 roundings correlate: (axes, x, q)."""
+    rng = np.random.default_rng(3)
+    g = np.linspace(-2.5, 2.5, 9).astype(np.float32)
+    if name == "rocksalt":  # 10^3 lattice of alternating +-1 e, 2.82 A spacing
+        idx = np.stack(np.meshgrid(*[np.arange(10)] * 3, indexing="ij"), -1).reshape(-1, 3)
+        x = ((idx - 4.5) * 2.82 + 0.37).astype(np.float32)
+        q = np.where(idx.sum(1) % 2 == 0, 1.0, -1.0).astype(np.float32)
+        return (g, g, g), x, q
+    if name == "shell":  # 3,000 equal charges on a 9 A sphere: E ~ 0 inside
+        m = 3000
+        k = np.arange(m) + 0.5
+        phi = np.arccos(1 - 2 * k / m)
+        th = np.pi * (1 + 5 ** 0.5) * k
+        x = 9.0 * np.stack([np.cos(th) * np.sin(phi), np.sin(th) * np.sin(phi), np.cos(phi)], 1)
+        return (g, g, g), x.astype(np.float32), np.full(m, 0.3, np.float32)
+    if name.startswith("mirror"):  # same-sign clusters on both sides of a field zero
+        mult = int(name[6:])
+        c1, c2 = np.array([3.0, 0.0, 0.0]), np.array([-3.0, 0.0, 0.0])
+        jit = 0.05 if mult < 64 else 0.0
+        s1 = c1 + jit * rng.standard_normal((64 // mult, 3))
+        s2 = c2 + jit * rng.standard_normal((64 // mult, 3))
+        x = np.concatenate([np.repeat(s1, mult, 0), np.repeat(s2, mult, 0)]).astype(np.float32)
+        q = np.concatenate([np.full(64, 0.37), np.full(64, 0.37 * 1.1)]).astype(np.float32)
+        ax = np.linspace(0.0715 - 0.03, 0.0715 + 0.03, 61).astype(np.float32)
+        ay = np.linspace(-0.2, 0.2, 5).astype(np.float32)
+        return (ax, ay, ay), x, q
+    from pycpet_amd import workloads as W
+
+    x, q = W.clustered(600, 10, 20.0, 1.5, -3.0, 5)
+    return (g, g, g), x, q
+
+
+@pytest.mark.parametrize("name", ["rocksalt", "shell", "mirror1", "mirror2", "mirror4",
+                                  "mirror16", "clustered"])
+def test_auto_precision_adversarial(gpu_ctx, oracle_mod, name):
+    """The auto mode's accepted FP32 points stay within 1e-5 of |E| (strict,
+    no RMS floor) on layouts built to correlate rounding: a rock-salt
+    lattice, a shell of equal charges, same-sign clusters (up to 16
+    coincident copies of a charge) on both sides of a field zero, and a
+    clustered frame.  Points whose FP64 estimate u A / |E| is within 10 % of
+    the check's 4e-6 are certainly unflagged and are held to the strict
+    bound; every point meets the usual 1e-5 pointwise metric."""
+    from pycpet_amd import ops
+
+    axes, x, q = _adversarial(name)
+    print(f"case adversarial {name} N={q.size} M={np.prod([a.size for a in axes])}", flush=True)
+    E, _ = ops.field_grid(*axes, x, q, ctx=gpu_ctx)
+    refined = gpu_ctx.refined_points()
+    pts = mesh(axes)
+    E64, _ = oracle_mod.field_f64(pts, x, q)
+    d = pts[:, None, :].astype(np.float64) - x[None].astype(np.float64)
+    A = 14.3996451 * (np.abs(q.astype(np.float64))[None] / (d ** 2).sum(2)).sum(1)
+    nE = np.linalg.norm(E64, axis=1)
+    err = np.linalg.norm(E - E64, axis=1)
+    sure = 2.0 ** -24 * A <= 0.9 * 4e-6 * nE
+    print(f"{name}: refined {refined}, strict points {sure.sum()}, "
+          f"max rel {(err[sure] / nE[sure]).max() if sure.any() else 0:.2e}", flush=True)
+    assert sure.any() or refined == pts.shape[0]  # a shell cancels everywhere: all refined
+    assert (err[sure] <= 1e-5 * nE[sure]).all()
+    assert rel_err(E, E64)[0] <= TOL
