@@ -329,7 +329,11 @@ int cpet_ctx_refined_points(cpet_ctx *ctx, long long *n);
  *   along z, b = charge splits), kind 1 = points field (a = points per lane,
  *   b = splits), kind 2 = topo (a = waves per workgroup, b = workgroups),
  *   kind 3 = batch topo (a = waves per workgroup, b = persistent
- *   workgroups), kind 4 = batch volume (a = points per lane). */
+ *   workgroups), kind 4 = batch volume (a = points per lane), kind 5 =
+ *   streamline pair term (0 or 1: the direct form; 2: the expanded
+ *   far-field form for frames whose charges are all far from the walk
+ *   region, the direct form for the others; plain charges, 4 or more waves
+ *   per workgroup). */
 int cpet_set_tuning(cpet_ctx *ctx, int kind, int a, int b);
 
 /* ------------------------------------------------------ cluster step

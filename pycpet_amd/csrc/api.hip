@@ -515,7 +515,7 @@ int cpet_ctx_refined_points(cpet_ctx* ctx, long long* n) {
 }
 
 int cpet_set_tuning(cpet_ctx* ctx, int kind, int a, int b) {
-  if (!ctx || kind < 0 || kind > 4 || a < 0 || b < 0) {
+  if (!ctx || kind < 0 || kind > 5 || a < 0 || b < 0) {
     set_error("cpet_set_tuning: bad arguments");
     return CPET_ERR_ARG;
   }

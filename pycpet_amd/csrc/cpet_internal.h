@@ -80,7 +80,7 @@ struct cpet_ctx {
   hipStream_t own = nullptr;     // the context's own stream
   bool own_stream = false;
   int num_cus = 0;
-  int tune[5][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};  // autotuner overrides
+  int tune[6][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};  // autotuner overrides
   std::mutex mu;
   std::vector<cpet::DevBuf> bufs;  // indexed by Slot
   // host-side pinned staging for small results

@@ -876,9 +876,9 @@ int field_launch(cpet_ctx* ctx, bool grid_mode, int nx, int ny, int nz,
     // size (21^3..101^3, 5k..30k charges, against K = 1, 3, 4, 6, 8), where
     // sharing dx, dy over more points saves less than the coarser waves
     // cost.  The potential alone shares four FP64 operations per charge
-    // (dx, dy, dx^2 + dy^2) against ten per pair, and K = 4 is faster there
-    // (101^3 x 30k: 10.1 against 10.7 ms).
-    K = forced_k > 0 ? forced_k : (what == kWhatPhi ? 4 : 2);
+    // (dx, dy, dx^2 + dy^2) against ten per pair, and more points per lane
+    // are faster there (101^3 x 30k: 10.9 / 10.2 / 9.9 ms at K = 2 / 4 / 8).
+    K = forced_k > 0 ? forced_k : (what == kWhatPhi ? (nz >= 40 ? 8 : 4) : 2);
     nzc = (nz + K - 1) / K;
     lanes = (long long)nx * ny * nzc;
   } else {

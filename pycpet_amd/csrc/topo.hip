@@ -220,7 +220,7 @@ struct PropArgs {
   int32_t* steps;   // (L)
   int32_t* exited;  // (L)
   unsigned int* counter;
-  const int* far_unsafe;   // 0: every charge is far from the walk region
+  const int* far_unsafe;   // null (no far-field form), or 0: every charge is far
 };
 
 // Streams source records [r0, r1) (float4; r0 and r1 multiples of four) to
@@ -516,12 +516,18 @@ struct IsF64<Rk4<Base>> : IsF64<Base> {};
 // workload (charges at least 3 A from the box) the field errors of the two
 // forms against FP64 are the same size (max 4.9e-7 and 4.4e-7 of |E| in a
 // float32 emulation over 60 points).
+//
+// The form is opt-in (cpet_set_tuning kind 5, a = 2): on that workload it
+// runs 155 ms per frame against the direct form's 146 ms (16 waves x 256
+// workgroups), because its two records per charge double the scalar loads
+// and the SGPRs that hold them, which costs more than the two VALU
+// operations it saves.
 constexpr float kFarRatio = 3.0f;
 
 struct ChargeSrcFar {
   static constexpr int kRecPerSrc = 2;
   static constexpr int kComp = 4;   // S0, Sx, Sy, Sz
-  static constexpr int kGroup = 4;  // records per scalar-load group: 2 charges (8 overflow the SGPRs)
+  static constexpr int kGroup = 8;  // records per scalar-load group: 4 charges
   __device__ static void slice(float px, float py, float pz, const float4* __restrict__ ch,
                                int r0, int r1, double (&e)[4]) {
     // -2 p (exact) and |p|^2, once per evaluation
@@ -786,7 +792,7 @@ __device__ __forceinline__ void run_slots(const float4* __restrict__ ch, const F
 // kernel's stores.
 template <int S, class Src>
 __global__ __launch_bounds__(64 * S, 8) void topo_propagate_kernel(
-    const float4* __restrict__ ch, const float4* __restrict__ far, PropArgs a) {
+    const float4* __restrict__ ch, PropArgs a) {
   __shared__ double part[kPartBufs(S) * S * 4 * 64];
   __shared__ double esum[4 * 64];
   __shared__ int nsid[2][64];
@@ -805,18 +811,44 @@ __global__ __launch_bounds__(64 * S, 8) void topo_propagate_kernel(
   f.steps = a.steps;
   f.exited = a.exited;
   f.counter = a.counter;
-  if constexpr (std::is_same<Src, ChargeSrc>::value && S >= 4) {
-    if (far && __builtin_amdgcn_readfirstlane(*a.far_unsafe) == 0) {
-      constexpr int rps = ChargeSrcFar::kRecPerSrc, gran = ChargeSrcFar::kGroup;
-      const int nrec = used_records(far, 0, a.npad * rps, rps, lane, gran);
-      wave_records(0, __builtin_amdgcn_readfirstlane(nrec), wave, S, f.r0, f.r1, gran);
-      run_slots<S, ChargeSrcFar>(far, f, blockIdx.x * 64 + lane, part, esum, nsid, lane, wave);
-      return;
-    }
-  }
+  if (a.far_unsafe && __builtin_amdgcn_readfirstlane(*a.far_unsafe) == 0)
+    return;  // every charge is far: topo_far_kernel runs this launch's work
   const int nrec = used_records(ch, 0, a.npad * Src::kRecPerSrc, Src::kRecPerSrc, lane);
   wave_records(0, __builtin_amdgcn_readfirstlane(nrec), wave, S, f.r0, f.r1);
   run_slots<S, Src>(ch, f, blockIdx.x * 64 + lane, part, esum, nsid, lane, wave);
+}
+
+// The far-field form (ChargeSrcFar) of the single-frame kernel, launched
+// beside topo_propagate_kernel; exactly one of the two does the work, as the
+// device flag decides.  Its own kernel, so that its four-charge scalar-load
+// groups (32 SGPRs per buffer) get the SGPR budget of a 4-wave occupancy
+// target: one 16-wave workgroup per CU runs 4 waves per SIMD anyway.
+template <int S>
+__global__ __launch_bounds__(64 * S, 4) void topo_far_kernel(const float4* __restrict__ far,
+                                                           PropArgs a) {
+  __shared__ double part[kPartBufs(S) * S * 4 * 64];
+  __shared__ double esum[4 * 64];
+  __shared__ int nsid[2][64];
+  if (__builtin_amdgcn_readfirstlane(*a.far_unsafe) != 0) return;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  FrameView f;
+  f.L = a.L;
+  f.starts = a.starts;
+  f.n_iter = a.n_iter;
+  f.order = a.order;
+  f.h = a.h;
+  f.d0 = a.d0;
+  f.d1 = a.d1;
+  f.d2 = a.d2;
+  f.fin = a.fin;
+  f.steps = a.steps;
+  f.exited = a.exited;
+  f.counter = a.counter;
+  constexpr int rps = ChargeSrcFar::kRecPerSrc, gran = ChargeSrcFar::kGroup;
+  const int nrec = used_records(far, 0, a.npad * rps, rps, lane, gran);
+  wave_records(0, __builtin_amdgcn_readfirstlane(nrec), wave, S, f.r0, f.r1, gran);
+  run_slots<S, ChargeSrcFar>(far, f, blockIdx.x * 64 + lane, part, esum, nsid, lane, wave);
 }
 
 // Many frames in one persistent launch.  Workgroups claim assignments
@@ -839,27 +871,35 @@ struct BatchTopoArgs {
   const int32_t* order;       // (sum L) per-frame longest-first local ids
   const int32_t* assign;      // frame id per assignment
   int n_assign;
-  unsigned int* assign_counter;
-  const int* far_unsafe;      // (F) 0: frame f's charges are far from its walk region
+  unsigned int* assign_counter;  // [0]: direct-form kernel, [1]: far-form kernel
+  const int* far_unsafe;      // (F) 0: frame f's charges are far from its walk region, or null
 };
 
-template <int S>
-__global__ __launch_bounds__(64 * S, 8) void topo_batch_kernel(const float4* __restrict__ ch,
-                                                            const float4* __restrict__ far,
-                                                            BatchTopoArgs a) {
+// The batch body for one form: FAR = 0 runs the frames that use the direct
+// pair term, FAR = 1 those whose charges are all far from their walk region
+// (far_unsafe[f] == 0).  The two forms are separate kernels (with their own
+// occupancy targets, see topo_far_kernel), each walking the whole
+// assignment list with its own counter and skipping the other form's
+// frames, so every frame runs in exactly one of them.
+template <int S, bool FAR>
+__device__ __forceinline__ void batch_body(const float4* __restrict__ src, BatchTopoArgs a) {
   __shared__ double part[kPartBufs(S) * S * 4 * 64];
   __shared__ double esum[4 * 64];
   __shared__ int nsid[2][64];
   __shared__ int cur;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  unsigned int* counter = a.assign_counter + (FAR ? 1 : 0);
   while (true) {
     __syncthreads();  // previous assignment's LDS reads are done
-    if (threadIdx.x == 0) cur = (int)atomicAdd(a.assign_counter, 1u);
+    if (threadIdx.x == 0) cur = (int)atomicAdd(counter, 1u);
     __syncthreads();
     const int as = __builtin_amdgcn_readfirstlane(cur);
     if (as >= a.n_assign) break;
     const int fr = a.assign[as];
+    const bool far_frame =
+        a.far_unsafe && __builtin_amdgcn_readfirstlane(a.far_unsafe[fr]) == 0;
+    if (far_frame != FAR) continue;
     const int s0 = a.sl_off[fr];
     FrameView f;
     f.L = a.sl_off[fr + 1] - s0;
@@ -875,17 +915,29 @@ __global__ __launch_bounds__(64 * S, 8) void topo_batch_kernel(const float4* __r
     f.exited = a.exited + s0;
     f.counter = a.sl_counter + fr;
     const int c0 = a.chg_off[fr], nc = a.chg_off[fr + 1] - c0;
-    if (S >= 4 && far && __builtin_amdgcn_readfirstlane(a.far_unsafe[fr]) == 0) {
+    if constexpr (FAR) {
       constexpr int rps = ChargeSrcFar::kRecPerSrc, gran = ChargeSrcFar::kGroup;
-      const int nrec = used_records(far, rps * c0, rps * nc, rps, lane, gran);
+      const int nrec = used_records(src, rps * c0, rps * nc, rps, lane, gran);
       wave_records(rps * c0, __builtin_amdgcn_readfirstlane(nrec), wave, S, f.r0, f.r1, gran);
-      run_slots<S, ChargeSrcFar>(far, f, -1, part, esum, nsid, lane, wave);
+      run_slots<S, ChargeSrcFar>(src, f, -1, part, esum, nsid, lane, wave);
     } else {
-      const int nrec = used_records(ch, c0, nc, 1, lane);
+      const int nrec = used_records(src, c0, nc, 1, lane);
       wave_records(c0, __builtin_amdgcn_readfirstlane(nrec), wave, S, f.r0, f.r1);
-      run_slots<S, ChargeSrc>(ch, f, -1, part, esum, nsid, lane, wave);
+      run_slots<S, ChargeSrc>(src, f, -1, part, esum, nsid, lane, wave);
     }
   }
+}
+
+template <int S>
+__global__ __launch_bounds__(64 * S, 8) void topo_batch_kernel(const float4* __restrict__ ch,
+                                                            BatchTopoArgs a) {
+  batch_body<S, false>(ch, a);
+}
+
+template <int S>
+__global__ __launch_bounds__(64 * S, 4) void topo_batch_far_kernel(
+    const float4* __restrict__ far, BatchTopoArgs a) {
+  batch_body<S, true>(far, a);
 }
 
 // ---- 3. FP64 epilogue -------------------------------------------------
@@ -1124,7 +1176,14 @@ __global__ __launch_bounds__(kEpiWaves * kEpiLanes) void topo_batch_epilogue_ker
 template <int S, class Src>
 hipError_t launch_prop(hipStream_t st, int G, const float4* ch, const float4* far,
                        const PropArgs& a) {
-  hipLaunchKernelGGL((topo_propagate_kernel<S, Src>), dim3(G), dim3(64 * S), 0, st, ch, far, a);
+  if constexpr (std::is_same<Src, ChargeSrc>::value && S >= 4) {
+    if (far) {
+      hipLaunchKernelGGL((topo_far_kernel<S>), dim3(G), dim3(64 * S), 0, st, far, a);
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
+  }
+  hipLaunchKernelGGL((topo_propagate_kernel<S, Src>), dim3(G), dim3(64 * S), 0, st, ch, a);
   return hipGetLastError();
 }
 
@@ -1203,7 +1262,7 @@ int topo_run(cpet_ctx* ctx, int N, int L, const float* starts,
   if (S <= 0) S = 16;  // 16 waves x 64 slots: fastest measured (S 4/8/16, G 256..2048)
   const float4* d_far = nullptr;
   const int* d_far_unsafe = nullptr;
-  if (src == kSrcCharge && S >= 4) {
+  if (src == kSrcCharge && S >= 4 && ctx->tune[5][0] == 2) {
     void* fb = nullptr;
     if ((rc = ctx_buf(ctx, kSlotFar, (size_t)npad * 2 * sizeof(float4) + 256, &fb))) return rc;
     int* flag = (int*)((char*)fb + (size_t)npad * 2 * sizeof(float4));
@@ -1409,7 +1468,7 @@ int topo_batch_run(cpet_ctx* ctx, int F, const int32_t* n_starts, const float* s
   if ((rc = upload_table(ctx, kTabTopoBlk, tw + o_blk, blk.data(), blk.size() * sizeof(int32_t)))) return rc;
   // ... and the counters, histograms and edges the call writes
   const size_t o_cnt = 0;
-  const size_t o_hist = ((F + 1) * sizeof(uint32_t) + 255) & ~(size_t)255;
+  const size_t o_hist = ((F + 2) * sizeof(uint32_t) + 255) & ~(size_t)255;
   const size_t o_edges = (o_hist + (size_t)F * nbins * sizeof(uint32_t) + 255) & ~(size_t)255;
   const size_t o_bk = (o_edges + (hist ? (size_t)(nb_d + nb_c + 2) * sizeof(double) : 0) + 255) &
                       ~(size_t)255;
@@ -1419,8 +1478,8 @@ int topo_batch_run(cpet_ctx* ctx, int F, const int32_t* n_starts, const float* s
   if ((rc = ctx_buf(ctx, kSlotBatchWork, total, &work))) return rc;
   char* w = (char*)work;
   unsigned int* d_cnt = (unsigned int*)(w + o_cnt);
-  hipLaunchKernelGGL(fill_u32_kernel, dim3((F + 1 + 255) / 256), dim3(256), 0, st, d_cnt,
-                     (long long)(F + 1), 0u);
+  hipLaunchKernelGGL(fill_u32_kernel, dim3((F + 2 + 255) / 256), dim3(256), 0, st, d_cnt,
+                     (long long)(F + 2), 0u);
   unsigned int* d_hist = nullptr;
   const double *d_ed = nullptr, *d_ec = nullptr;
   if (hist) {
@@ -1455,7 +1514,7 @@ int topo_batch_run(cpet_ctx* ctx, int F, const int32_t* n_starts, const float* s
   // far-field records of every frame and the per-frame safety flags
   const float4* d_far = nullptr;
   const int* d_far_unsafe = nullptr;
-  if (S >= 4) {
+  if (S >= 4 && ctx->tune[5][0] == 2) {
     const int total = chg_off[F];
     void* fb = nullptr;
     if ((rc = ctx_buf(ctx, kSlotFar, (size_t)total * 2 * sizeof(float4) + (size_t)F * 4 + 256,
@@ -1490,12 +1549,21 @@ int topo_batch_run(cpet_ctx* ctx, int F, const int32_t* n_starts, const float* s
   ba.assign_counter = d_cnt + F;
   ba.far_unsafe = d_far_unsafe;
   if ((rc = prof_mark(ctx, 0))) return rc;
+  if (d_far) {  // the far-form frames (none when every flag is set)
+    switch (S) {
+      case 4: hipLaunchKernelGGL((topo_batch_far_kernel<4>), dim3(G), dim3(256), 0, st, d_far, ba); break;
+      case 8: hipLaunchKernelGGL((topo_batch_far_kernel<8>), dim3(G), dim3(512), 0, st, d_far, ba); break;
+      case 16: hipLaunchKernelGGL((topo_batch_far_kernel<16>), dim3(G), dim3(1024), 0, st, d_far, ba); break;
+      default: break;
+    }
+    CPET_HIP_TRY(hipGetLastError());
+  }
   switch (S) {
-    case 1: hipLaunchKernelGGL((topo_batch_kernel<1>), dim3(G), dim3(64), 0, st, ch, d_far, ba); break;
-    case 2: hipLaunchKernelGGL((topo_batch_kernel<2>), dim3(G), dim3(128), 0, st, ch, d_far, ba); break;
-    case 4: hipLaunchKernelGGL((topo_batch_kernel<4>), dim3(G), dim3(256), 0, st, ch, d_far, ba); break;
-    case 8: hipLaunchKernelGGL((topo_batch_kernel<8>), dim3(G), dim3(512), 0, st, ch, d_far, ba); break;
-    case 16: hipLaunchKernelGGL((topo_batch_kernel<16>), dim3(G), dim3(1024), 0, st, ch, d_far, ba); break;
+    case 1: hipLaunchKernelGGL((topo_batch_kernel<1>), dim3(G), dim3(64), 0, st, ch, ba); break;
+    case 2: hipLaunchKernelGGL((topo_batch_kernel<2>), dim3(G), dim3(128), 0, st, ch, ba); break;
+    case 4: hipLaunchKernelGGL((topo_batch_kernel<4>), dim3(G), dim3(256), 0, st, ch, ba); break;
+    case 8: hipLaunchKernelGGL((topo_batch_kernel<8>), dim3(G), dim3(512), 0, st, ch, ba); break;
+    case 16: hipLaunchKernelGGL((topo_batch_kernel<16>), dim3(G), dim3(1024), 0, st, ch, ba); break;
     default:
       set_error("unsupported waves per workgroup %d (1,2,4,8,16)", S);
       return CPET_ERR_ARG;

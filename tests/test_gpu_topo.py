@@ -229,3 +229,52 @@ def test_f64_mode_vs_oracle(gpu_ctx, oracle_mod):
         assert dd.max() <= 1e-10, dd.max()
     with pytest.raises(ValueError):
         ops.topo(c.starts, c.n_iter, c.h, c.dims, c.x, c.q, prec="kahan", ctx=gpu_ctx)
+
+
+def _far_frame(N, L, cap, seed, r_in=6.0, r_out=30.0):
+    """Charges in a shell r_in..r_out A around a +-1.5 A box, h = 1e-3:
+    with r_in >= 6 every charge passes the far-field guard
+    (|dims + h| + |c| <= 3 dist(c, box))."""
+    from pycpet_amd import workloads as W
+
+    rng = np.random.default_rng(seed)
+    u = rng.standard_normal((N, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    x = (u * rng.uniform(r_in, r_out, (N, 1))).astype(np.float32)
+    q = rng.uniform(-1, 1, N).astype(np.float32)
+    dims = np.array([1.5, 1.5, 1.5], np.float32)
+    starts, n_iter, _ = W.topo_starts(L, dims, 0.001, cap, seed=seed + 42, start_seed=seed)
+    return dict(x=x, q=q, starts=starts, n_iter=n_iter, h=0.001, dims=dims)
+
+
+def test_far_field_form_opt_in(gpu_ctx, oracle_mod):
+    """The expanded far-field pair term (tuning kind 5, a = 2), single frame
+    and batched: on frames whose charges all pass its guard it meets the same
+    oracle bar as the direct form (and is a different sum: not bit-identical
+    to it); a frame with charges next to the box keeps the direct form
+    (bit-identical)."""
+    from pycpet_amd import batch, ops
+
+    fa = _far_frame(600, 384, 1500, 21)
+    fb = _far_frame(500, 256, 300, 22, r_in=3.0)  # charges from 3 A: some fail the guard
+    f64, s64, e64 = oracle_mod.topo_f64(fa["starts"], fa["n_iter"], fa["h"], fa["dims"], fa["x"],
+                                        fa["q"])
+    run = lambda f: ops.topo(f["starts"], f["n_iter"], f["h"], f["dims"], f["x"], f["q"],
+                             ctx=gpu_ctx)
+    direct_a, direct_b = run(fa), run(fb)
+    direct_batch = batch.batch_topo([fa, fb], ctx=gpu_ctx)
+    gpu_ctx.set_tuning(5, 2, 0)
+    try:
+        far_a, far_b = run(fa), run(fb)
+        far_batch = batch.batch_topo([fa, fb], ctx=gpu_ctx)
+    finally:
+        gpu_ctx.set_tuning(5, 0, 0)
+    print("case far-field shells 600x384 (r >= 6) and 500x256 (r >= 3), h=1e-3", flush=True)
+    for r in (far_a, far_batch[0]):
+        compare(r, f64, s64, e64, "far")
+        assert not np.array_equal(r.feats, direct_a.feats)
+    compare(direct_batch[0], f64, s64, e64, "direct batch")
+    for r, d in ((far_b, direct_b), (far_batch[1], direct_batch[1])):
+        np.testing.assert_array_equal(r.feats, d.feats)
+        np.testing.assert_array_equal(r.steps, d.steps)
+        np.testing.assert_array_equal(r.exited, d.exited)
