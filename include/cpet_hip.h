@@ -226,6 +226,29 @@ int cpet_topo(cpet_ctx *ctx, int N, int L, const float *starts,
               int32_t *exited, int nb_d, const double *edges_d, int nb_c,
               const double *edges_c, uint32_t *hist, uint32_t flags);
 
+/* Bytes of the fixed device scratch array of cpet_topo_noalloc (one copy
+ * per GPU, held until the process ends). */
+#define CPET_NOALLOC_SCRATCH_BYTES (32u << 20)
+
+/* cpet_topo that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Every
+ * array but dims (host, 3 floats) is a device pointer of the current device
+ * (CPET_F_DEVICE is implied); the work is queued on hip_stream (NULL: the
+ * null stream), and the call waits for it unless flags has CPET_F_NO_SYNC.
+ * All scratch comes from the library's fixed array of
+ * CPET_NOALLOC_SCRATCH_BYTES: about 52 bytes per streamline plus 48 per
+ * charge (e.g. 100,000 streamlines over 50,000 charges take 7.6 MB); a
+ * call that needs more returns CPET_ERR_BUDGET.  Default precision and the
+ * Euler walk only (CPET_F_PREC_F64 and CPET_F_TOPO_RK4 are rejected).
+ * Calls on one GPU share the array: calls that overlap on the device (on
+ * different streams) race.  Results equal cpet_topo's bitwise. */
+int cpet_topo_noalloc(void *hip_stream, int N, int L, const float *starts,
+                      const int32_t *n_iter, float h, const float *dims, const float *x,
+                      const float *q, double *feats, int32_t *steps, int32_t *exited, int nb_d,
+                      const double *edges_d, int nb_c, const double *edges_c, uint32_t *hist,
+                      uint32_t flags);
+
 /* L dipole streamlines: cpet_topo with point dipoles x (N,3), mu (N,3) as
  * the sources (calc_field_base_dipole, math_module.c:335-371; walk and
  * features of thread_operation_dipole, :593-659).  The pair term is the

@@ -85,6 +85,8 @@ SIGNATURES = {
     "cpet_field_grid": [c_void_p, c_int, c_int, c_int, P, P, P, c_int, P, P, P, P, c_int, c_uint32],
     "cpet_topo": [c_void_p, c_int, c_int, P, P, c_float, P, P, P, P, P, P,
                   c_int, P, c_int, P, P, c_uint32],
+    "cpet_topo_noalloc": [c_void_p, c_int, c_int, P, P, c_float, P, P, P, P, P, P,
+                          c_int, P, c_int, P, P, c_uint32],
     "cpet_set_tuning": [c_void_p, c_int, c_int, c_int],
     "cpet_topo_dipole": [c_void_p, c_int, c_int, P, P, c_float, P, P, P, P, P, P, c_uint32],
     "cpet_ctx_set_profiling": [c_void_p, c_int],
