@@ -237,8 +237,8 @@ int cpet_topo(cpet_ctx *ctx, int N, int L, const float *starts,
  * (CPET_F_DEVICE is implied); the work is queued on hip_stream (NULL: the
  * null stream), and the call waits for it unless flags has CPET_F_NO_SYNC.
  * All scratch comes from the library's fixed array of
- * CPET_NOALLOC_SCRATCH_BYTES: about 52 bytes per streamline plus 48 per
- * charge (e.g. 100,000 streamlines over 50,000 charges take 7.6 MB); a
+ * CPET_NOALLOC_SCRATCH_BYTES: about 60 bytes per streamline plus 16 per
+ * charge (e.g. 100,000 streamlines over 50,000 charges take 6.8 MB); a
  * call that needs more returns CPET_ERR_BUDGET.  Default precision and the
  * Euler walk only (CPET_F_PREC_F64 and CPET_F_TOPO_RK4 are rejected).
  * Calls on one GPU share the array: calls that overlap on the device (on

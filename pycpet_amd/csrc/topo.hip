@@ -216,9 +216,7 @@ struct PropArgs {
   const int32_t* n_iter;
   const int32_t* order;
   double h, d0, d1, d2;
-  double* fin;      // (L,3) final points
-  int32_t* steps;   // (L)
-  int32_t* exited;  // (L)
+  double4* res;     // (L) final point, steps and stop kind (ResultRec)
   unsigned int* counter;
   const int* far_unsafe;   // null (no far-field form), or 0: every charge is far
 };
@@ -597,6 +595,17 @@ __global__ __launch_bounds__(256) void far_pack_kernel(int total, const float4* 
 
 enum SlotState : int { kActive = 0, kNeed = 1, kDone = 2 };
 
+// Result record of one streamline, written once when it stops: the final
+// point (FP64) and, in the bit pattern of the fourth double, the step count
+// (low word) and the stop kind (high word; 1 = left the box).  One aligned
+// 32-byte record per streamline, so each scattered store fills one whole
+// 32-byte HBM sector instead of parts of three arrays' sectors; the
+// epilogue, which reads the records in streamline order, writes the step
+// counts and stop kinds out coalesced.
+__device__ __forceinline__ double4 result_rec(double x, double y, double z, int steps, int out) {
+  return make_double4(x, y, z, __hiloint2double(out, steps));
+}
+
 // One frame's streamline set as the slot loop sees it.  Streamline indices
 // are local to the frame; outputs are indexed the same way.
 struct FrameView {
@@ -605,9 +614,7 @@ struct FrameView {
   const int32_t* n_iter;   // (L)
   const int32_t* order;    // queue position -> streamline, or null (identity)
   double h, d0, d1, d2;
-  double* fin;             // (L,3)
-  int32_t* steps;          // (L)
-  int32_t* exited;         // (L)
+  double4* res;            // (L) result records
   unsigned int* counter;   // next queue position
   int r0, r1;              // this wave's source records (float4)
 };
@@ -641,13 +648,7 @@ __device__ __forceinline__ void run_slots(const float4* __restrict__ ch, const F
   double qx = 0.0, qy = 0.0, qz = 0.0, ax = 0.0, ay = 0.0, az = 0.0;
 
   auto finish = [&](int out) {
-    if (wave == 0) {
-      f.fin[3 * sid] = px;
-      f.fin[3 * sid + 1] = py;
-      f.fin[3 * sid + 2] = pz;
-      f.steps[sid] = step;
-      f.exited[sid] = out;
-    }
+    if (wave == 0) f.res[sid] = result_rec(px, py, pz, step, out);
     st = kNeed;
   };
   auto load = [&](int q) {
@@ -807,9 +808,7 @@ __global__ __launch_bounds__(64 * S, 8) void topo_propagate_kernel(
   f.d0 = a.d0;
   f.d1 = a.d1;
   f.d2 = a.d2;
-  f.fin = a.fin;
-  f.steps = a.steps;
-  f.exited = a.exited;
+  f.res = a.res;
   f.counter = a.counter;
   if (a.far_unsafe && __builtin_amdgcn_readfirstlane(*a.far_unsafe) == 0)
     return;  // every charge is far: topo_far_kernel runs this launch's work
@@ -841,9 +840,7 @@ __global__ __launch_bounds__(64 * S, 4) void topo_far_kernel(const float4* __res
   f.d0 = a.d0;
   f.d1 = a.d1;
   f.d2 = a.d2;
-  f.fin = a.fin;
-  f.steps = a.steps;
-  f.exited = a.exited;
+  f.res = a.res;
   f.counter = a.counter;
   constexpr int rps = ChargeSrcFar::kRecPerSrc, gran = ChargeSrcFar::kGroup;
   const int nrec = used_records(far, 0, a.npad * rps, rps, lane, gran);
@@ -864,9 +861,7 @@ struct BatchTopoArgs {
   const float* starts;        // (sum L, 3)
   const int32_t* n_iter;      // (sum L)
   const double* hdims;        // (F,4): h, d0, d1, d2
-  double* fin;                // (sum L, 3)
-  int32_t* steps;             // (sum L)
-  int32_t* exited;            // (sum L)
+  double4* res;               // (sum L) result records
   unsigned int* sl_counter;   // (F) per-frame queue positions, start at 0
   const int32_t* order;       // (sum L) per-frame longest-first local ids
   const int32_t* assign;      // frame id per assignment
@@ -910,9 +905,7 @@ __device__ __forceinline__ void batch_body(const float4* __restrict__ src, Batch
     f.d0 = a.hdims[4 * fr + 1];
     f.d1 = a.hdims[4 * fr + 2];
     f.d2 = a.hdims[4 * fr + 3];
-    f.fin = a.fin + 3LL * s0;
-    f.steps = a.steps + s0;
-    f.exited = a.exited + s0;
+    f.res = a.res + s0;
     f.counter = a.sl_counter + fr;
     const int c0 = a.chg_off[fr], nc = a.chg_off[fr + 1] - c0;
     if constexpr (FAR) {
@@ -945,9 +938,11 @@ __global__ __launch_bounds__(64 * S, 4) void topo_batch_far_kernel(
 struct EpiArgs {
   int L, npad;
   const float* starts;
-  const double* fin;
+  const double4* res;  // result records of the propagate kernel
   double h;
   double* feats;
+  int32_t* steps;
+  int32_t* exited;
   const double* ed;
   int nbd;
   const double* ec;
@@ -1083,20 +1078,24 @@ __device__ __forceinline__ double curve_f64(const double (&a)[3],
 // or not: the steps have barriers); wave 0 stores the features.
 template <class Src>
 __device__ __forceinline__ void epilogue_one(const float4* __restrict__ ch, int npad,
-                                             const float* starts, const double* fin,
-                                             double h, double* feats, int l, int L,
+                                             const float* starts, const double4* res,
+                                             double h, double* feats, int32_t* steps,
+                                             int32_t* exited, int l, int L,
                                              const EpiArgs& a, unsigned int* lh,
                                              unsigned int* hist, EpiShared& sh) {
   const int wave = threadIdx.x / kEpiLanes, lane = threadIdx.x % kEpiLanes;
   const bool valid = l < L;
+  const double4 rec = valid ? res[l] : make_double4(0.0, 0.0, 0.0, 0.0);
   double p[2][3], q1[2][3], q2[2][3];
-  for (int i = 0; i < 3; ++i) {
-    p[0][i] = valid ? (double)starts[3 * l + i] : 0.0;
-    p[1][i] = valid ? fin[3 * l + i] : 0.0;
-  }
+  for (int i = 0; i < 3; ++i) p[0][i] = valid ? (double)starts[3 * l + i] : 0.0;
+  p[1][0] = rec.x;
+  p[1][1] = rec.y;
+  p[1][2] = rec.z;
   step2_f64<Src>(p, q1, ch, npad, h, sh, wave, lane);
   step2_f64<Src>(q1, q2, ch, npad, h, sh, wave, lane);
   if (!valid || wave != 0) return;
+  steps[l] = __double2loint(rec.w);
+  exited[l] = __double2hiint(rec.w);
   double kap[2];
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
@@ -1136,7 +1135,8 @@ __global__ __launch_bounds__(kEpiWaves * kEpiLanes) void topo_epilogue_kernel(
     __syncthreads();
   }
   const int l = blockIdx.x * kEpiLanes + threadIdx.x % kEpiLanes;
-  epilogue_one<Src>(ch, a.npad, a.starts, a.fin, a.h, a.feats, l, a.L, a, lh, a.hist, sh);
+  epilogue_one<Src>(ch, a.npad, a.starts, a.res, a.h, a.feats, a.steps, a.exited, l, a.L, a, lh,
+                    a.hist, sh);
   if (a.hist && a.lds_hist) {
     __syncthreads();
     for (int i = threadIdx.x; i < nbins; i += blockDim.x)
@@ -1164,8 +1164,8 @@ __global__ __launch_bounds__(kEpiWaves * kEpiLanes) void topo_batch_epilogue_ker
   const int l = b * kEpiLanes + threadIdx.x % kEpiLanes;
   unsigned int* hist = a.hist ? a.hist + (size_t)fr * nbins : nullptr;
   epilogue_one<ChargeSrc>(ch + chg_off[fr], chg_off[fr + 1] - chg_off[fr], a.starts + 3LL * s0,
-                          a.fin + 3LL * s0, hdims[4 * fr], a.feats + 2LL * s0, l, L, a, lh,
-                          hist, sh);
+                          a.res + s0, hdims[4 * fr], a.feats + 2LL * s0, a.steps + s0,
+                          a.exited + s0, l, L, a, lh, hist, sh);
   if (a.hist && a.lds_hist) {
     __syncthreads();
     for (int i = threadIdx.x; i < nbins; i += blockDim.x)
@@ -1216,13 +1216,13 @@ int topo_run(cpet_ctx* ctx, int N, int L, const float* starts,
   if ((rc = stage_in(ctx, kSlotPts, starts, (size_t)L * 3 * sizeof(float), dev, &dstarts))) return rc;
   if ((rc = stage_in(ctx, kSlotIter, n_iter, (size_t)L * sizeof(int32_t), dev, &diter))) return rc;
 
-  // state: fin (L*3 double) | feats (L*2 double) | steps (L) | exited (L)
+  // state: result records (L double4) | feats (L*2 double) | steps (L) | exited (L)
   void* state = nullptr;
-  const size_t st_bytes = (size_t)L * (5 * sizeof(double) + 2 * sizeof(int32_t));
+  const size_t st_bytes = (size_t)L * (6 * sizeof(double) + 2 * sizeof(int32_t));
   if ((rc = ctx_buf(ctx, kSlotTopoState, st_bytes, &state))) return rc;
-  double* d_fin = (double*)state;
-  double* d_feats = dev ? feats : d_fin + 3 * (size_t)L;
-  int32_t* d_steps = (int32_t*)(d_fin + 5 * (size_t)L);
+  double4* d_res = (double4*)state;
+  double* d_feats = dev ? feats : (double*)(d_res + L);
+  int32_t* d_steps = (int32_t*)((double*)(d_res + L) + 2 * (size_t)L);
   int32_t* d_exited = d_steps + L;
   if (dev && steps) d_steps = steps;
   if (dev && exited) d_exited = exited;
@@ -1308,9 +1308,7 @@ int topo_run(cpet_ctx* ctx, int N, int L, const float* starts,
   pa.d0 = dims[0];
   pa.d1 = dims[1];
   pa.d2 = dims[2];
-  pa.fin = d_fin;
-  pa.steps = d_steps;
-  pa.exited = d_exited;
+  pa.res = d_res;
   pa.counter = d_counter;
   pa.far_unsafe = d_far_unsafe;
   if (S != 1 && S != 2 && S != 4 && S != 8 && S != 16) {
@@ -1334,9 +1332,11 @@ int topo_run(cpet_ctx* ctx, int N, int L, const float* starts,
   ea.L = L;
   ea.npad = npad;
   ea.starts = (const float*)dstarts;
-  ea.fin = d_fin;
+  ea.res = d_res;
   ea.h = (double)h;
   ea.feats = d_feats;
+  ea.steps = d_steps;
+  ea.exited = d_exited;
   ea.ed = d_ed;
   ea.nbd = nb_d;
   ea.ec = d_ec;
@@ -1413,13 +1413,16 @@ int topo_batch_run(cpet_ctx* ctx, int F, const int32_t* n_starts, const float* s
   if ((rc = stage_in(ctx, kSlotPts, starts, (size_t)Lt * 3 * sizeof(float), dev, &dstarts))) return rc;
   if ((rc = stage_in(ctx, kSlotIter, n_iter, (size_t)Lt * sizeof(int32_t), dev, &diter))) return rc;
   void* state = nullptr;
-  if ((rc = ctx_buf(ctx, kSlotTopoState, (size_t)Lt * (5 * sizeof(double) + 2 * sizeof(int32_t)),
+  // state: result records (Lt double4) | feats (Lt*2 double) | steps (Lt) | exited (Lt)
+  if ((rc = ctx_buf(ctx, kSlotTopoState, (size_t)Lt * (6 * sizeof(double) + 2 * sizeof(int32_t)),
                     &state)))
     return rc;
-  double* d_fin = (double*)state;
-  double* d_feats = dev ? feats : d_fin + 3 * (size_t)Lt;
-  int32_t* d_steps = (dev && steps) ? steps : (int32_t*)(d_fin + 5 * (size_t)Lt);
-  int32_t* d_exited = (dev && exited) ? exited : (int32_t*)(d_fin + 5 * (size_t)Lt) + Lt;
+  double4* d_res = (double4*)state;
+  double* d_feats = dev ? feats : (double*)(d_res + Lt);
+  int32_t* d_steps =
+      (dev && steps) ? steps : (int32_t*)((double*)(d_res + Lt) + 2 * (size_t)Lt);
+  int32_t* d_exited =
+      (dev && exited) ? exited : (int32_t*)((double*)(d_res + Lt) + 2 * (size_t)Lt) + Lt;
 
   const int S = ctx->tune[3][0] > 0 ? ctx->tune[3][0] : 16;  // as the single-frame kernel: same results
   const int G0 = ctx->tune[3][1] > 0 ? ctx->tune[3][1] : ctx->num_cus * (S >= 16 ? 1 : 2);
@@ -1539,9 +1542,7 @@ int topo_batch_run(cpet_ctx* ctx, int F, const int32_t* n_starts, const float* s
   ba.starts = (const float*)dstarts;
   ba.n_iter = (const int32_t*)diter;
   ba.hdims = (const double*)(tw + o_hd);
-  ba.fin = d_fin;
-  ba.steps = d_steps;
-  ba.exited = d_exited;
+  ba.res = d_res;
   ba.sl_counter = d_cnt;
   ba.order = d_ord;
   ba.assign = (const int32_t*)(tw + o_as);
@@ -1574,9 +1575,11 @@ int topo_batch_run(cpet_ctx* ctx, int F, const int32_t* n_starts, const float* s
   ea.L = 0;
   ea.npad = 0;
   ea.starts = (const float*)dstarts;
-  ea.fin = d_fin;
+  ea.res = d_res;
   ea.h = 0.0;
   ea.feats = d_feats;
+  ea.steps = d_steps;
+  ea.exited = d_exited;
   ea.ed = d_ed;
   ea.nbd = nb_d;
   ea.ec = d_ec;

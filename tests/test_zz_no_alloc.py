@@ -64,7 +64,7 @@ def test_noalloc_refuses_oversized_calls(gpu_ctx):
     lib = nat.load()
     dev = torch.device("cuda:0")
     T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
-    c = W.small_topo(N=64, L=700_000, cap=1, seed=13)  # ~36 MB of scratch
+    c = W.small_topo(N=64, L=700_000, cap=1, seed=13)  # ~42 MB of scratch
     rc, *_ = _noalloc(lib, torch.cuda.current_stream(dev).cuda_stream, c, T,
                       (np.linspace(0, 1, 3), np.linspace(0, 1, 3)))
     assert rc == nat.CPET_ERR_BUDGET
