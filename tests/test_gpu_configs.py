@@ -1,6 +1,7 @@
 """The streamline, batching and serving configurations at their full sizes.
 
   c1  5,000 charges, 100,000 starts, h = 1e-3 A, n_iter capped at 2,000,
+      Euler (the reference's walk) and RK4 (the configuration's),
       with the 20 x 20 (distance, curvature) histogram whose range comes
       from the FP64 features' min / max (the range rule of make_histograms,
       CPET/utils/calculator.py:702-707, and get_hist_grid,
@@ -83,6 +84,45 @@ def test_c1_full_size_with_histogram(gpu_ctx, oracle_mod):
           flush=True)
     assert same.mean() >= 0.99
     assert np.abs(Hn - H64).sum() <= 2 * moved
+
+
+def test_c1_rk4_full_size_with_histogram(gpu_ctx, oracle_mod):
+    """c1 as the configuration names it, with RK4 steps: the FP64-sum RK4
+    walk pinned to the RK4 oracle on a seeded sample, its features giving
+    the histogram range, and the FP32 RK4 walk with its histogram against it
+    over all 100,000 streamlines."""
+    from pycpet_amd import ops, workloads as W
+
+    c = W.topo_5k()
+    print(f"case c1 rk4 topo_5k N={c.N} L={c.L} with 20x20 histogram", flush=True)
+    r64 = ops.topo(c.starts, c.n_iter, c.h, c.dims, c.x, c.q, prec="f64", integrator="rk4",
+                   ctx=gpu_ctx)
+    idx = np.random.default_rng(6).choice(c.L, 24, replace=False)
+    o64, os64, oe64 = oracle_mod.topo_rk4_f64(c.starts[idx], c.n_iter[idx], c.h, c.dims, c.x,
+                                              c.q)
+    np.testing.assert_array_equal(r64.steps[idx], os64)
+    np.testing.assert_array_equal(r64.exited[idx], oe64)
+    np.testing.assert_allclose(r64.feats[idx], o64, rtol=1e-9, atol=1e-12)
+    f64 = r64.feats
+    ed = np.linspace(f64[:, 0].min(), f64[:, 0].max(), 21)
+    ec = np.linspace(f64[:, 1].min(), f64[:, 1].max(), 21)
+    r = ops.topo(c.starts, c.n_iter, c.h, c.dims, c.x, c.q, edges=(ed, ec), integrator="rk4",
+                 ctx=gpu_ctx)
+    fr = r.feats.astype(np.float32).astype(np.float64)
+    Hn, _, _ = np.histogram2d(fr[:, 0], fr[:, 1], bins=[ed, ec])
+    np.testing.assert_array_equal(r.hist, Hn.astype(np.uint32))
+    m = (r.steps == r64.steps) & (r.exited == r64.exited)
+    dd = np.abs(r.feats[m, 0] - f64[m, 0]) / np.maximum(np.abs(f64[m, 0]), 1e-12)
+    dk = np.abs(r.feats[m, 1] - f64[m, 1]) / np.maximum(np.abs(f64[m, 1]), 1e-6)
+    print(f"c1 rk4: match {m.mean():.5f} dist {dd.max():.2e} curv {dk.max():.2e}", flush=True)
+    assert m.mean() >= 0.99 and dd.max() <= FEAT_TOL and dk.max() <= FEAT_TOL
+    f64r = f64.astype(np.float32).astype(np.float64)
+    same = m & (_bins(fr[:, 0], ed) == _bins(f64r[:, 0], ed)) & \
+        (_bins(fr[:, 1], ec) == _bins(f64r[:, 1], ec))
+    Hs32, _, _ = np.histogram2d(fr[same, 0], fr[same, 1], bins=[ed, ec])
+    Hs64, _, _ = np.histogram2d(f64r[same, 0], f64r[same, 1], bins=[ed, ec])
+    np.testing.assert_array_equal(Hs32, Hs64)
+    assert same.mean() >= 0.99
 
 
 def test_c3_md_frames_at_config_size(gpu_ctx, oracle_mod):
