@@ -1,8 +1,8 @@
 """ORACLE — test infrastructure, not product code.
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
-import this module, and only as the checker / the CPU baseline.  The product
-(pycpet_amd/) never imports it.
+import this module, and only as the reference results are compared with or
+as the CPU baseline.  The product (pycpet_amd/) never imports it.
 
 Contents
   field_f64 / topo_f64      ctypes front-ends of liboracle.so (cpet_oracle.c),
