@@ -3,7 +3,8 @@
  *
  * A from-scratch CPU restatement of PyCPET's two hot paths, used only as the
  * reference that results are compared with, by tests/, by
- * __graft_entry__.smoke() and by bench.py's cpu_baseline leg.  The product path (pycpet_amd/) never loads this library.
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg.  The product
+ * path (pycpet_amd/) never loads this library.
  *
  * Two families:
  *   *_f64  FP64 restatement of the reference semantics on the FP32 inputs the
