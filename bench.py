@@ -30,7 +30,9 @@ charges) and `volume_esp_30k` (the same grid, potential only; both
 --no-volume), `topo_f64` (the FP64 run: its speed and the FP32 path's match
 fraction and feature errors against it; --no-f64 drops the line) and
 `topo_rk4` (the same streamlines integrated with RK4, four field
-evaluations per step; --no-rk4).
+evaluations per step; --no-rk4).  `aux` (--no-aux): point-dipole
+streamlines, frame I/O and the cluster step, each beside its CPU
+counterpart.
 """
 from __future__ import annotations
 
@@ -142,6 +144,162 @@ def cpu_baseline_grids():
     return out
 
 
+def _write_pdb(path, x, q):
+    """A frame in the fixed-column PDB layout the reference parser reads
+    (coordinates in columns 31-54, charge in 56-64; CPET/utils/io.py:574-636)."""
+    with open(path, "w") as fh:
+        for i in range(len(q)):
+            fh.write(f"ATOM  {(i % 99999) + 1:5d} C{i % 5:<3d} ALA A{1 + (i // 5) % 9999:4d}    "
+                     f"{x[i, 0]:8.3f}{x[i, 1]:8.3f}{x[i, 2]:8.3f} {q[i]:8.4f}  1.8240       C  \n")
+        fh.write("END\n")
+
+
+def aux_rows(ctx, T, case, cpu: bool, cores: int):
+    """The rows that feed and follow the two hot paths, each timed on the GPU
+    (or native host code) beside its CPU counterpart:
+      dipole_topo_5k  point-dipole streamlines (thread_operation_dipole,
+                      CPET/utils/math_module.c:593-659): the 5,000 positions of
+                      the main workload as dipoles, the same 100,000 starts;
+      frame_io        PDB parse (CPET/utils/io.py:574-636) of a 200,000-atom
+                      frame, .top writing (np.savetxt, CPET/source/CPET.py:123)
+                      and reading of 1,000,000 rows;
+      cluster_step    make_histograms + construct_distance_matrix
+                      (CPET/utils/calculator.py:596-718,1000-1015) over 256
+                      frames of 20,000 features."""
+    import tempfile
+
+    import torch
+
+    from pycpet_amd import clusterhist as CH, io as cio, ops
+
+    out = {}
+    # -- dipole streamlines
+    rng = np.random.default_rng(11)
+    mu = (0.3 * rng.standard_normal((case.N, 3))).astype(np.float32)
+    st, it, dx, dmu = T(case.starts), T(case.n_iter), T(case.x), T(mu)
+    ops.topo_dipole(st, it, case.h, case.dims, dx, dmu, ctx=ctx)
+    torch.cuda.synchronize()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(2):
+        r = ops.topo_dipole(st, it, case.h, case.dims, dx, dmu, ctx=ctx)
+        kms.append(ctx.phase_times(3)[1])
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / 2
+    dsteps = r.steps.cpu().numpy()
+    pairs = float((dsteps.astype(np.int64) + 4).sum()) * case.N
+    row = {"pair_evals_per_s": pairs / wall, "ms_per_step": 1e3 * wall,
+           "kernel_ms": float(np.mean(kms)), "streamlines_per_s": case.L / wall,
+           "euler_steps": int(dsteps.astype(np.int64).sum())}
+    if cpu:
+        from concurrent.futures import ThreadPoolExecutor
+
+        from oracle import oracle as O
+
+        if O.Ref.available():
+            ref = O.Ref()
+            order = np.random.default_rng(124).permutation(case.L)
+            done, pos = [], 0
+            t1 = time.perf_counter()
+            with ThreadPoolExecutor(max_workers=cores) as ex:
+                while time.perf_counter() - t1 < 4.0 and pos < case.L:
+                    batch = order[pos:pos + 4 * cores]
+                    list(ex.map(lambda i: ref.thread_operation_dipole(
+                        case.starts[i], case.n_iter[i], case.h, case.dims, case.x, mu), batch))
+                    done.extend(batch.tolist())
+                    pos += len(batch)
+            dt = time.perf_counter() - t1
+            cp = float((dsteps[done].astype(np.int64) + 4).sum()) * case.N
+            row["cpu_reference"] = {"pair_evals_per_s": cp / dt, "cores": cores,
+                                    "sample": f"{len(done)} streamlines, {dt:.1f} s"}
+    out["dipole_topo_5k"] = row
+
+    with tempfile.TemporaryDirectory() as tmp:
+        # -- frame I/O
+        n_atoms = 200_000
+        xa = np.round(rng.uniform(-60, 60, (n_atoms, 3)), 3)
+        qa = np.round(rng.normal(0, 0.4, n_atoms), 4)
+        pdb = os.path.join(tmp, "frame.pdb")
+        _write_pdb(pdb, xa, qa)
+        mb = os.path.getsize(pdb) / 1e6
+        cio.parse_pdb(pdb, get_charges=True)
+        t1 = time.perf_counter()
+        nat_out = cio.parse_pdb(pdb, get_charges=True)
+        t_nat = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        py_out = cio._parse_pdb_text(pdb, True, "float32")
+        t_py = time.perf_counter() - t1
+        same = bool(np.array_equal(nat_out[0], py_out[0]) and np.array_equal(nat_out[1], py_out[1]))
+        rows = rng.standard_normal((1_000_000, 2))
+        top = os.path.join(tmp, "a.top")
+        ref_top = os.path.join(tmp, "b.top")
+        t1 = time.perf_counter()
+        cio.write_rows(top, rows, fmt="%.18e")
+        t_w = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        np.savetxt(ref_top, rows)
+        t_ws = time.perf_counter() - t1
+        with open(top, "rb") as fa, open(ref_top, "rb") as fb:
+            same_top = fa.read() == fb.read()
+        t1 = time.perf_counter()
+        back = CH.read_topo_file(top)
+        t_r = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        back_np = np.loadtxt(top)
+        t_rl = time.perf_counter() - t1
+        out["frame_io"] = {
+            "pdb_atoms": n_atoms, "pdb_mb": mb,
+            "pdb_parse_native_s": t_nat, "pdb_parse_python_s": t_py,
+            "pdb_parse_speedup": t_py / t_nat, "pdb_same_arrays": same,
+            "top_rows": len(rows), "top_write_native_s": t_w, "top_write_savetxt_s": t_ws,
+            "top_write_speedup": t_ws / t_w, "top_bytes_identical": bool(same_top),
+            "top_read_native_s": t_r, "top_read_loadtxt_s": t_rl,
+            "top_read_speedup": t_rl / t_r,
+            "top_read_equal": bool(np.array_equal(back, back_np)),
+            "python_side": "pdb: the line-by-line restatement of the reference parser in "
+                           "pycpet_amd.io; .top: numpy.savetxt / numpy.loadtxt"}
+
+    # -- cluster step: per-frame features from a mixture, like .top files
+    F, Lf = 256, 20_000
+    feats = []
+    for f in range(F):
+        g = np.random.default_rng(500 + f)
+        feats.append(np.column_stack([np.abs(g.normal(1.0 + 0.002 * f, 0.3, Lf)),
+                                      np.abs(g.normal(5.0, 2.0, Lf))]))
+    CH.histograms_from_features(feats[:2], ctx=ctx)
+    t1 = time.perf_counter()
+    edges = CH.histogram_edges(feats)  # the shared bins (scipy iqr), as the reference
+    hist = CH.histograms_from_features(feats, edges=edges, ctx=ctx)
+    t_h = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    dm = CH.construct_distance_matrix(hist, ctx=ctx)
+    t_d = time.perf_counter() - t1
+    row = {"frames": F, "features_per_frame": Lf, "bins": [len(edges[0]) - 1, len(edges[1]) - 1],
+           "histograms_s": t_h, "chi2_matrix_s": t_d}
+    if cpu:
+        from oracle import oracle as O
+
+        # the reference steps are slow (about 80 ms per frame for the
+        # histograms, a Python call per frame pair for the matrix): time them
+        # on the first 32 / 16 frames and scale linearly / by frame pairs
+        nh, nd_ = 32, 16
+        t1 = time.perf_counter()
+        oh, _ = O.cluster_histograms(feats[:nh])
+        t_oh = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        O.chi2_matrix(oh[:nd_])
+        t_od = time.perf_counter() - t1
+        row["cpu_reference"] = {
+            "histograms_s_scaled_to_all": t_oh * F / nh,
+            "chi2_matrix_s_scaled_to_all": t_od * (F * (F - 1)) / (nd_ * (nd_ - 1)),
+            "sample": f"histograms of {nh} frames in {t_oh:.2f} s, the matrix of {nd_} "
+                      f"frames in {t_od:.2f} s, scaled to {F} frames",
+            "how": "the reference's steps with numpy.histogram2d / scipy iqr and sklearn "
+                   "pairwise_distances over the chi2 function, one core"}
+    out["cluster_step"] = row
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,6 +310,7 @@ def main() -> int:
     ap.add_argument("--no-volume", action="store_true")
     ap.add_argument("--no-rk4", action="store_true")
     ap.add_argument("--no-f64", action="store_true")
+    ap.add_argument("--no-aux", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
@@ -336,9 +495,12 @@ def main() -> int:
             "achieved_tflops": FLOP_PER_PAIR * 4 * rsteps * case.N / (np.mean(kms) * 1e-3) / 1e12,
         }
 
+    avail = len(os.sched_getaffinity(0))
+    cores = min(16, avail)
+    if not args.no_aux:
+        out["aux"] = aux_rows(ctx, T, case, not args.no_cpu_baseline, cores)
+
     if not args.no_cpu_baseline and int(os.environ.get("RANK", "0")) == 0:
-        avail = len(os.sched_getaffinity(0))
-        cores = min(16, avail)
         out["cpu_baseline"] = cpu_baseline(case, steps_np, args.cpu_seconds, cores)
         out["cpu_baseline"]["host_nproc"] = os.cpu_count()
         out["cpu_baseline"]["host_cpus_usable"] = avail

@@ -13,7 +13,14 @@
 //                 Values are formatted from the double value of the stored
 //                 float16/float32/float64, columns joined by ' ', rows by '\n'.
 
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <algorithm>
 #include <cctype>
+#include <charconv>
 #include <cerrno>
 #include <cmath>
 #include <cstdint>
@@ -21,6 +28,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <string_view>
 #include <thread>
 #include <vector>
 
@@ -43,6 +51,95 @@ bool read_file(const char* path, std::string& out) {
   std::fclose(f);
   return got == out.size();
 }
+
+// A read-only view of a whole file: mapped when it is a regular file (no
+// copy of the text), else read into memory.
+struct TextView {
+  const char* data = nullptr;
+  size_t size = 0;
+  void* map = nullptr;
+  std::string copy;
+  bool open(const char* path) {
+    const int fd = ::open(path, O_RDONLY);
+    if (fd < 0) return false;
+    struct stat st;
+    if (::fstat(fd, &st) == 0 && S_ISREG(st.st_mode) && st.st_size > 0) {
+      void* m = ::mmap(nullptr, (size_t)st.st_size, PROT_READ, MAP_PRIVATE, fd, 0);
+      if (m != MAP_FAILED) {
+        ::madvise(m, (size_t)st.st_size, MADV_SEQUENTIAL);
+        map = m;
+        data = (const char*)m;
+        size = (size_t)st.st_size;
+        ::close(fd);
+        return true;
+      }
+    }
+    ::close(fd);
+    if (!read_file(path, copy)) return false;
+    data = copy.data();
+    size = copy.size();
+    return true;
+  }
+  ~TextView() {
+    if (map) ::munmap(map, size);
+  }
+};
+
+// A text cut into up to 16 pieces at line boundaries (whole lines, in file
+// order) for per-line work on threads; small texts stay in one piece.
+struct Pieces {
+  std::string_view s;
+  std::vector<size_t> cut;
+  int n = 1;
+  explicit Pieces(std::string_view text) : s(text) {
+    const int hw = (int)std::thread::hardware_concurrency();
+    n = std::max(1, std::min({hw, 16, (int)(s.size() >> 20) + 1}));
+    cut.assign(n + 1, s.size());
+    cut[0] = 0;
+    for (int t = 1; t < n; ++t) {
+      const size_t p = std::max(cut[t - 1], s.size() * t / n);
+      const size_t j = p < s.size() ? s.find('\n', p) : std::string_view::npos;
+      cut[t] = j == std::string_view::npos ? s.size() : j + 1;
+    }
+  }
+  // fn(line, len) for every line of piece t; CRLF reads as LF, as text-mode
+  // reading turns CRLF into LF
+  template <typename Fn>
+  void lines(int t, Fn fn) const {
+    size_t i = cut[t];
+    while (i < cut[t + 1]) {
+      size_t j = s.find('\n', i);
+      if (j == std::string_view::npos || j >= cut[t + 1]) j = cut[t + 1];
+      size_t e = j;
+      if (e > i && s[e - 1] == '\r') --e;
+      fn(s.data() + i, e - i);
+      i = j + 1;
+    }
+  }
+  // work(t) for every piece, piece 0 on the calling thread
+  template <typename Work>
+  void run(Work work) const {
+    std::vector<std::thread> th;
+    for (int t = 1; t < n; ++t) th.emplace_back(work, t);
+    work(0);
+    for (auto& x : th) x.join();
+  }
+  // per-piece line and record counts (records: lines where rec(l, len))
+  template <typename Rec>
+  void count(Rec rec, std::vector<int64_t>& lines_, std::vector<int64_t>& recs) const {
+    lines_.assign(n, 0);
+    recs.assign(n, 0);
+    run([&](int t) {
+      int64_t nl = 0, nr = 0;  // counted locally: neighbouring entries share a cache line
+      lines(t, [&](const char* l, size_t len) {
+        ++nl;
+        nr += rec(l, len) ? 1 : 0;
+      });
+      lines_[t] = nl;
+      recs[t] = nr;
+    });
+  }
+};
 
 // Calls fn(line_start, line_len) for every line of text.
 template <typename Fn>
@@ -106,13 +203,32 @@ bool parse_int(Field f, int32_t* v) {
   return true;
 }
 
-// Python's float() accepts decimal literals, inf/infinity and nan; strtod also
-// takes hexadecimal ("0x1p3"), which float() rejects.
+// Python's float() accepts decimal literals with an optional sign,
+// inf/infinity and nan.  std::from_chars reads the same decimal forms,
+// correctly rounded, without a leading '+' and without hexadecimal; a value
+// beyond the double range, which from_chars reports as out of range, goes
+// to strtod, which rounds it to +-inf or a (sub)normal or zero as float()
+// does.
 bool parse_float(Field f, double* v) {
-  char buf[64];  // "%.18e" values are 24-25 characters
-  if (!terminated(f, buf, sizeof(buf)) || std::strpbrk(buf, "xX")) return false;
+  const char* b = f.p;
+  const char* e = f.p + f.n;
+  if (b == e) return false;
+  if (*b == '+') {
+    ++b;
+    if (b == e || *b == '+' || *b == '-') return false;
+  }
+  double x = 0.0;
+  const auto r = std::from_chars(b, e, x);
+  if (r.ptr != e) return false;
+  if (r.ec == std::errc()) {
+    *v = x;
+    return true;
+  }
+  if (r.ec != std::errc::result_out_of_range) return false;
+  char buf[64];
+  if (!terminated(f, buf, sizeof(buf))) return false;
   char* end = nullptr;
-  const double x = std::strtod(buf, &end);  // correctly rounded (glibc)
+  x = std::strtod(buf, &end);  // correctly rounded (glibc)
   if (*end != '\0') return false;
   *v = x;
   return true;
@@ -206,14 +322,21 @@ int cpet_pdb_scan(const char* path, int* n_atoms) {
     set_error("cpet_pdb_scan: NULL argument");
     return CPET_ERR_ARG;
   }
-  std::string s;
-  if (!read_file(path, s)) {
+  TextView tv;
+  if (!tv.open(path)) {
     set_error("cannot read %s", path);
     return CPET_ERR_ARG;
   }
-  int n = 0;
-  for_lines(s, [&](const char* l, size_t len) { n += is_record(l, len) ? 1 : 0; });
-  *n_atoms = n;
+  const Pieces pc(std::string_view(tv.data, tv.size));
+  std::vector<int64_t> lines, recs;
+  pc.count(is_record, lines, recs);
+  int64_t n = 0;
+  for (int64_t r : recs) n += r;
+  if (n > INT32_MAX) {
+    set_error("%s has more than %d atom records", path, INT32_MAX);
+    return CPET_ERR_ARG;
+  }
+  *n_atoms = (int)n;
   return CPET_OK;
 }
 
@@ -225,48 +348,72 @@ int cpet_pdb_parse(const char* path, int n_atoms, int get_charges, double* xyz, 
     set_error("cpet_pdb_parse: bad arguments");
     return CPET_ERR_ARG;
   }
-  std::string s;
-  if (!read_file(path, s)) {
+  TextView tv;
+  if (!tv.open(path)) {
     set_error("cannot read %s", path);
     return CPET_ERR_ARG;
   }
-  int i = 0, lineno = 0, rc = CPET_OK;
-  for_lines(s, [&](const char* l, size_t n) {
-    ++lineno;
-    if (rc != CPET_OK || !is_record(l, n)) return;
-    if (!is_ascii(l, n)) {
-      set_error("non-ASCII bytes in %s line %d", path, lineno);
-      rc = CPET_ERR_ARG;
-      return;
-    }
-    if (i >= n_atoms) {
-      set_error("%s has more than %d atom records", path, n_atoms);
-      rc = CPET_ERR_ARG;
-      return;
-    }
-    double v[4];
-    if (!parse_int(field(l, n, 6, 11), &atom_number[i]) ||
-        !parse_int(field(l, n, 22, 26), &resnum[i]) || !parse_float(field(l, n, 30, 38), &v[0]) ||
-        !parse_float(field(l, n, 38, 46), &v[1]) || !parse_float(field(l, n, 46, 54), &v[2]) ||
-        (get_charges && !parse_float(field(l, n, 55, 64), &v[3]))) {
-      set_error("%s line %d: invalid literal in a fixed-width field", path, lineno);
-      rc = CPET_ERR_ARG;
-      return;
-    }
-    xyz[3 * i] = v[0];
-    xyz[3 * i + 1] = v[1];
-    xyz[3 * i + 2] = v[2];
-    if (get_charges) q[i] = v[3];
-    copy_fixed(atom_type + 4 * (size_t)i, 4, field(l, n, 12, 16));
-    copy_fixed(resname + 3 * (size_t)i, 3, field(l, n, 17, 20));
-    chain[i] = n > 21 ? l[21] : '\0';
-    ++i;
-  });
-  if (rc == CPET_OK && i != n_atoms) {
-    set_error("%s has %d atom records, expected %d", path, i, n_atoms);
-    rc = CPET_ERR_ARG;
+  // counts per piece, then every piece parses its records from its first
+  // record index on; the earliest failing line is reported
+  const Pieces pc(std::string_view(tv.data, tv.size));
+  std::vector<int64_t> lines, recs;
+  pc.count(is_record, lines, recs);
+  int64_t total = 0;
+  for (int64_t r : recs) total += r;
+  if (total > n_atoms) {
+    set_error("%s has more than %d atom records", path, n_atoms);
+    return CPET_ERR_ARG;
   }
-  return rc;
+  if (total != n_atoms) {
+    set_error("%s has %lld atom records, expected %d", path, (long long)total, n_atoms);
+    return CPET_ERR_ARG;
+  }
+  std::vector<int64_t> rec0(pc.n, 0), line0(pc.n, 0), bad_line(pc.n, -1);
+  std::vector<int> bad_kind(pc.n, 0);  // 1: non-ASCII, 2: invalid literal
+  for (int t = 1; t < pc.n; ++t) {
+    rec0[t] = rec0[t - 1] + recs[t - 1];
+    line0[t] = line0[t - 1] + lines[t - 1];
+  }
+  pc.run([&](int t) {
+    int64_t i = rec0[t], k = 0;
+    pc.lines(t, [&](const char* l, size_t n) {
+      ++k;
+      if (bad_line[t] >= 0 || !is_record(l, n)) return;
+      if (!is_ascii(l, n)) {
+        bad_line[t] = line0[t] + k;
+        bad_kind[t] = 1;
+        return;
+      }
+      double v[4];
+      if (!parse_int(field(l, n, 6, 11), &atom_number[i]) ||
+          !parse_int(field(l, n, 22, 26), &resnum[i]) ||
+          !parse_float(field(l, n, 30, 38), &v[0]) || !parse_float(field(l, n, 38, 46), &v[1]) ||
+          !parse_float(field(l, n, 46, 54), &v[2]) ||
+          (get_charges && !parse_float(field(l, n, 55, 64), &v[3]))) {
+        bad_line[t] = line0[t] + k;
+        bad_kind[t] = 2;
+        return;
+      }
+      xyz[3 * i] = v[0];
+      xyz[3 * i + 1] = v[1];
+      xyz[3 * i + 2] = v[2];
+      if (get_charges) q[i] = v[3];
+      copy_fixed(atom_type + 4 * (size_t)i, 4, field(l, n, 12, 16));
+      copy_fixed(resname + 3 * (size_t)i, 3, field(l, n, 17, 20));
+      chain[i] = n > 21 ? l[21] : '\0';
+      ++i;
+    });
+  });
+  for (int t = 0; t < pc.n; ++t) {
+    if (bad_line[t] < 0) continue;
+    if (bad_kind[t] == 1)
+      set_error("non-ASCII bytes in %s line %lld", path, (long long)bad_line[t]);
+    else
+      set_error("%s line %lld: invalid literal in a fixed-width field", path,
+                (long long)bad_line[t]);
+    return CPET_ERR_ARG;
+  }
+  return CPET_OK;
 }
 
 // PQR records as parse_pqr reads them (CPET/utils/io.py:478-571): tokens of
@@ -288,68 +435,86 @@ int cpet_pqr_parse(const char* path, int n_atoms, int width, double* xyz, double
     set_error("cpet_pqr_parse: bad arguments");
     return CPET_ERR_ARG;
   }
-  std::string s;
-  if (!read_file(path, s)) {
+  TextView tv;
+  if (!tv.open(path)) {
     set_error("cannot read %s", path);
     return CPET_ERR_ARG;
   }
-  int i = 0, lineno = 0, rc = CPET_OK;
-  for_lines(s, [&](const char* l, size_t n) {
-    ++lineno;
-    if (rc != CPET_OK || !is_record(l, n)) return;
-    auto fail = [&](const char* why) {
-      set_error("%s line %d: %s", path, lineno, why);
-      rc = CPET_ERR_ARG;
-    };
-    if (!is_ascii(l, n)) return fail("non-ASCII bytes");
-    if (i >= n_atoms) return fail("more atom records than counted");
-    Field tok[8];
-    int nt = 0;
-    for (size_t k = 0; k < n && nt < 8;) {  // the first tokens of line.split()
-      while (k < n && py_space((unsigned char)l[k])) ++k;
-      size_t e = k;
-      while (e < n && !py_space((unsigned char)l[e])) ++e;
-      if (e > k) tok[nt++] = Field{l + k, e - k};
-      k = e;
-    }
-    const bool merged = nt > 0 && tok[0].n > 6;
-    const int res_ind = merged ? 4 : 5;
-    if (nt <= res_ind) return fail("too few fields");
-    const Field serial = merged ? Field{tok[0].p + 6, tok[0].n - 6} : tok[1];
-    const Field aname = merged ? tok[1] : tok[2];
-    const Field rname = merged ? tok[2] : tok[3];
-    const Field rtok = tok[res_ind].n > 4 ? Field{tok[res_ind - 1].p + 1, tok[res_ind - 1].n - 1}
-                                          : tok[res_ind];
-    int32_t rv = 0;
-    if (!parse_int(serial, &atom_number[i]) || !parse_int(rtok, &rv))
-      return fail("invalid serial or residue number");
-    if (aname.n >= (size_t)width || rname.n >= (size_t)width) return fail("token too long");
-    int shift = 0;
-    if (rv > 999) {
-      int digits = 0;
-      for (int64_t t = rv; t > 0; t /= 10) ++digits;
-      shift = digits - 3;
-    }
-    double v[4];
-    if (!parse_float(field(l, n, 30 + shift, 38 + shift), &v[0]) ||
-        !parse_float(field(l, n, 38 + shift, 46 + shift), &v[1]) ||
-        !parse_float(field(l, n, 46 + shift, 54 + shift), &v[2]) ||
-        !parse_float(field(l, n, 54 + shift, 61 + shift), &v[3]))
-      return fail("charge or coordinates are not usable numbers");
-    xyz[3 * i] = v[0];
-    xyz[3 * i + 1] = v[1];
-    xyz[3 * i + 2] = v[2];
-    q[i] = v[3];
-    resnum[i] = rv;
-    copy_fixed(atom_type + (size_t)width * i, (size_t)width, aname);
-    copy_fixed(resname + (size_t)width * i, (size_t)width, rname);
-    ++i;
-  });
-  if (rc == CPET_OK && i != n_atoms) {
-    set_error("%s has %d atom records, expected %d", path, i, n_atoms);
-    rc = CPET_ERR_ARG;
+  const Pieces pc(std::string_view(tv.data, tv.size));
+  std::vector<int64_t> lines, recs;
+  pc.count(is_record, lines, recs);
+  int64_t total = 0;
+  for (int64_t r : recs) total += r;
+  if (total != n_atoms) {
+    set_error("%s has %lld atom records, expected %d", path, (long long)total, n_atoms);
+    return CPET_ERR_ARG;
   }
-  return rc;
+  std::vector<int64_t> rec0(pc.n, 0), line0(pc.n, 0), bad_line(pc.n, -1);
+  std::vector<const char*> bad_why(pc.n, nullptr);
+  for (int t = 1; t < pc.n; ++t) {
+    rec0[t] = rec0[t - 1] + recs[t - 1];
+    line0[t] = line0[t - 1] + lines[t - 1];
+  }
+  pc.run([&](int t) {
+    int64_t i = rec0[t], k = 0;
+    pc.lines(t, [&](const char* l, size_t n) {
+      ++k;
+      if (bad_line[t] >= 0 || !is_record(l, n)) return;
+      auto fail = [&](const char* why) {
+        bad_line[t] = line0[t] + k;
+        bad_why[t] = why;
+      };
+      if (!is_ascii(l, n)) return fail("non-ASCII bytes");
+      Field tok[8];
+      int nt = 0;
+      for (size_t a = 0; a < n && nt < 8;) {  // the first tokens of line.split()
+        while (a < n && py_space((unsigned char)l[a])) ++a;
+        size_t e = a;
+        while (e < n && !py_space((unsigned char)l[e])) ++e;
+        if (e > a) tok[nt++] = Field{l + a, e - a};
+        a = e;
+      }
+      const bool merged = nt > 0 && tok[0].n > 6;
+      const int res_ind = merged ? 4 : 5;
+      if (nt <= res_ind) return fail("too few fields");
+      const Field serial = merged ? Field{tok[0].p + 6, tok[0].n - 6} : tok[1];
+      const Field aname = merged ? tok[1] : tok[2];
+      const Field rname = merged ? tok[2] : tok[3];
+      const Field rtok = tok[res_ind].n > 4
+                             ? Field{tok[res_ind - 1].p + 1, tok[res_ind - 1].n - 1}
+                             : tok[res_ind];
+      int32_t rv = 0;
+      if (!parse_int(serial, &atom_number[i]) || !parse_int(rtok, &rv))
+        return fail("invalid serial or residue number");
+      if (aname.n >= (size_t)width || rname.n >= (size_t)width) return fail("token too long");
+      int shift = 0;
+      if (rv > 999) {
+        int digits = 0;
+        for (int64_t d = rv; d > 0; d /= 10) ++digits;
+        shift = digits - 3;
+      }
+      double v[4];
+      if (!parse_float(field(l, n, 30 + shift, 38 + shift), &v[0]) ||
+          !parse_float(field(l, n, 38 + shift, 46 + shift), &v[1]) ||
+          !parse_float(field(l, n, 46 + shift, 54 + shift), &v[2]) ||
+          !parse_float(field(l, n, 54 + shift, 61 + shift), &v[3]))
+        return fail("charge or coordinates are not usable numbers");
+      xyz[3 * i] = v[0];
+      xyz[3 * i + 1] = v[1];
+      xyz[3 * i + 2] = v[2];
+      q[i] = v[3];
+      resnum[i] = rv;
+      copy_fixed(atom_type + (size_t)width * i, (size_t)width, aname);
+      copy_fixed(resname + (size_t)width * i, (size_t)width, rname);
+      ++i;
+    });
+  });
+  for (int t = 0; t < pc.n; ++t)
+    if (bad_line[t] >= 0) {
+      set_error("%s line %lld: %s", path, (long long)bad_line[t], bad_why[t]);
+      return CPET_ERR_ARG;
+    }
+  return CPET_OK;
 }
 
 int cpet_write_rows(const char* path, const char* header, const void* data, int dtype,
@@ -392,39 +557,54 @@ int cpet_read_rows(const char* path, int cols, double* out, int64_t cap, int64_t
     set_error("cpet_read_rows: bad arguments");
     return CPET_ERR_ARG;
   }
-  std::string s;
-  if (!read_file(path, s)) {
+  TextView tv;
+  if (!tv.open(path)) {
     set_error("cannot read %s", path);
     return CPET_ERR_ARG;
   }
-  int64_t n = 0;
-  int lineno = 0, rc = CPET_OK;
-  for_lines(s, [&](const char* l, size_t len) {
-    ++lineno;
-    if (rc != CPET_OK || (len > 0 && l[0] == '#')) return;
-    if (out) {
-      if (n >= cap) {
-        set_error("%s has more than %lld rows", path, (long long)cap);
-        rc = CPET_ERR_ARG;
-        return;
-      }
-      size_t k = 0;
+  // rows are the non-comment lines; pieces count them, then parse from their
+  // first row on, and the earliest failing line is reported
+  const Pieces pc(std::string_view(tv.data, tv.size));
+  std::vector<int64_t> lines, rows;
+  pc.count([](const char* l, size_t len) { return !(len > 0 && l[0] == '#'); }, lines, rows);
+  int64_t total = 0;
+  for (int64_t r : rows) total += r;
+  *n_rows = total;
+  if (!out) return CPET_OK;
+  if (total > cap) {
+    set_error("%s has more than %lld rows", path, (long long)cap);
+    return CPET_ERR_ARG;
+  }
+  std::vector<int64_t> row0(pc.n, 0), line0(pc.n, 0), bad(pc.n, -1);
+  for (int t = 1; t < pc.n; ++t) {
+    row0[t] = row0[t - 1] + rows[t - 1];
+    line0[t] = line0[t - 1] + lines[t - 1];
+  }
+  pc.run([&](int t) {
+    int64_t r = row0[t], k = 0;
+    pc.lines(t, [&](const char* l, size_t len) {
+      ++k;
+      if (bad[t] >= 0 || (len > 0 && l[0] == '#')) return;
+      size_t a = 0;
       for (int c = 0; c < cols; ++c) {
-        while (k < len && py_space((unsigned char)l[k])) ++k;
-        size_t e = k;
+        while (a < len && py_space((unsigned char)l[a])) ++a;
+        size_t e = a;
         while (e < len && !py_space((unsigned char)l[e])) ++e;
-        if (e == k || !parse_float(Field{l + k, e - k}, &out[n * cols + c])) {
-          set_error("%s line %d: expected %d numbers", path, lineno, cols);
-          rc = CPET_ERR_ARG;
+        if (e == a || !parse_float(Field{l + a, e - a}, &out[r * cols + c])) {
+          bad[t] = line0[t] + k;
           return;
         }
-        k = e;
+        a = e;
       }
-    }
-    ++n;
+      ++r;
+    });
   });
-  *n_rows = n;
-  return rc;
+  for (int t = 0; t < pc.n; ++t)
+    if (bad[t] >= 0) {
+      set_error("%s line %lld: expected %d numbers", path, (long long)bad[t], cols);
+      return CPET_ERR_ARG;
+    }
+  return CPET_OK;
 }
 
 }  // extern "C"

@@ -282,11 +282,16 @@ def _parse_pdb_text(pdb_file_path, get_charges, dtype):
 
 
 def _fixed_str(buf: np.ndarray, width: int) -> np.ndarray:
-    """NUL-padded byte records -> the unicode array np.array(list_of_str)
-    gives (item width = the longest string)."""
+    """NUL-padded ASCII byte records -> the unicode array np.array(list_of_str)
+    gives (item width = the longest string): the bytes widened to UCS-4 code
+    units and viewed as strings, which drop their trailing NULs as numpy's
+    do."""
     rec = buf.reshape(-1, width)
-    w = max(1, int((rec != 0).sum(axis=1).max())) if rec.shape[0] else 1
-    return buf.view(f"S{width}").reshape(-1).astype(f"U{w}")
+    if rec.shape[0] == 0:
+        return np.array([], dtype="U1")
+    nz = rec != 0
+    w = max(1, int((width - np.argmax(nz[:, ::-1], axis=1))[nz.any(axis=1)].max(initial=0)))
+    return np.ascontiguousarray(rec[:, :w]).astype(np.uint32).view(f"U{w}").reshape(-1)
 
 
 def parse_pdb(pdb_file_path, get_charges=False, float32=True):

@@ -304,3 +304,70 @@ def test_pqr_column_shift_equals_log10_rule():
         for n in (10 ** k - 1, 10 ** k, 10 ** k + 1):
             if n > 999 and n < 2 ** 31:
                 assert int(np.log10(n) - 2) == len(str(n)) - 3, n
+
+
+def test_large_files_parse_in_pieces(tmp_path):
+    """PDB and PQR files of several MiB, which the native parsers cut into
+    line-aligned pieces parsed on threads: identical arrays to the
+    line-by-line restatements, and an error in a late piece reported with
+    its file line number (the earliest failing line wins)."""
+    from pycpet_amd import _native as nat
+    from pycpet_amd.io import _parse_pdb_text, _parse_pqr_text, parse_pdb, parse_pqr
+
+    rng = np.random.default_rng(8)
+    n = 60_000
+    xs = rng.uniform(-60, 60, (n, 3)).round(3)
+    qs = rng.normal(0, 0.4, n).round(4)
+    recs = [f"ATOM  {(i % 99999) + 1:5d} C{i % 5:<3d} ALA {'AB'[i % 2]}{1 + (i // 5) % 9999:4d}    "
+            f"{xs[i, 0]:8.3f}{xs[i, 1]:8.3f}{xs[i, 2]:8.3f} {qs[i]:8.4f}  1.8240       C  \n"
+            for i in range(n)]
+    body = ["REMARK big frame\n"] + [r if i % 997 else "REMARK x\n" + r for i, r in enumerate(recs)]
+    p = tmp_path / "big.pdb"
+    p.write_text("".join(body) + "END\n")
+    assert p.stat().st_size > 4 << 20
+    a = parse_pdb(str(p), get_charges=True)
+    b = _parse_pdb_text(str(p), True, "float32")
+    b = (b[0], b[1]) + b[2:]
+    for u, v in zip(a, b):
+        assert u.dtype == v.dtype and u.shape == v.shape
+        np.testing.assert_array_equal(u, v)
+    lines = "".join(body).splitlines(keepends=True)
+    for bad_at in (len(lines) - 5, len(lines) // 2):
+        broken = lines[:]
+        broken[bad_at] = broken[bad_at][:32] + "x" + broken[bad_at][33:]
+        q = tmp_path / "broken.pdb"
+        q.write_text("".join(broken))
+        with pytest.raises(ValueError, match=f"line {bad_at + 1}:"):
+            parse_pdb(str(q), get_charges=True)
+    both = lines[:]
+    for at in (len(lines) - 5, len(lines) // 3):
+        both[at] = both[at][:32] + "x" + both[at][33:]
+    q = tmp_path / "broken2.pdb"
+    q.write_text("".join(both))
+    with pytest.raises(ValueError, match=f"line {len(lines) // 3 + 1}:"):
+        parse_pdb(str(q), get_charges=True)
+
+    pq = ["REMARK generated\n"]
+    for i in range(60_000):
+        rn = int(rng.choice([1, 42, 999, 1000, 1049, 9999, 10000, 12345]))
+        pq.append(_pqr_line(i + 1, "CA", "THR", "A", rn, rng.uniform(-99, 999, 3).round(3),
+                            float(rng.uniform(-1, 1)), merged=(i % 7 == 3 and i > 9)))
+    p = tmp_path / "big.pqr"
+    p.write_text("".join(pq))
+    assert p.stat().st_size > 3 << 20
+    a, b = parse_pqr(str(p)), _parse_pqr_text(str(p))
+    for u, v in zip(a, b):
+        assert u.dtype == v.dtype and u.shape == v.shape
+        np.testing.assert_array_equal(u, v)
+    lib = nat.load()
+    broken = pq[:]
+    at = len(pq) - 3
+    broken[at] = broken[at][:34] + "x" + broken[at][35:]
+    q = tmp_path / "broken.pqr"
+    q.write_text("".join(broken))
+    n_rec = 60_000
+    bufs = [np.empty(3 * n_rec), np.empty(n_rec), np.empty(n_rec, np.int32),
+            np.empty(n_rec, np.int32), np.empty(16 * n_rec, np.uint8),
+            np.empty(16 * n_rec, np.uint8)]  # kept alive across the call
+    rc = lib.cpet_pqr_parse(str(q).encode(), n_rec, 16, *[b.ctypes.data for b in bufs])
+    assert rc != 0 and f"line {at + 1}:" in nat.last_error(lib)

@@ -157,3 +157,44 @@ def test_fixed3_fast_path_bit_patterns(tmp_path):
         got = (tmp_path / "r.txt").read_text().splitlines()
         want = ["%.3f" % float(x) for x in arr]
         assert got == want
+
+
+def test_read_rows_numbers_as_python_float(tmp_path):
+    """The .top reader's number syntax and rounding are Python's float():
+    random %.18e / %.3f / %g values, signs, exponents at and past the double
+    range, subnormals, inf / nan spellings, and several pieces' worth of rows
+    (the reader splits large files over threads); malformed tokens raise with
+    the line number."""
+    import math
+
+    from pycpet_amd import clusterhist as CH, _native as nat
+
+    rng = np.random.default_rng(17)
+    toks = ["+1.5", "-0.0", "0", "1e308", "1.7976931348623157e308", "1e309", "-1e400",
+            "4.9e-324", "1e-320", "2.2250738585072014e-308", "1e-400", "inf", "-inf",
+            "Infinity", "+INF", "nan", "-NaN", ".5", "5.", "1E5", "-2.5E-3", "007", "1e+02",
+            "123456789012345678901234567890", "0.30000000000000004441"]
+    vals = rng.standard_normal(4000) * 10.0 ** rng.integers(-30, 30, 4000)
+    toks += [f"{v:.18e}" for v in vals] + [f"{v:.3f}" for v in vals[:500]] + \
+        [f"{v:g}" for v in vals[:500]] + [repr(float(v)) for v in vals[:500]]
+    if len(toks) % 2:
+        toks.append("1")
+    path = tmp_path / "n.top"
+    body = "# header\n" + "\n".join(f"{toks[i]} {toks[i + 1]}" for i in range(0, len(toks), 2))
+    reps = 1 + (3 << 20) // len(body)  # > 3 MiB: several pieces
+    path.write_text("\n".join([body] * reps) + "\n")
+    got = CH.read_topo_file(path)
+    want = np.array([float(t) for t in toks] * reps).reshape(-1, 2)
+    assert got.shape == want.shape
+    same = (got == want) | (np.isnan(got) & np.isnan(want))
+    assert same.all()
+    assert np.array_equal(np.signbit(got), np.signbit(want))
+    for bad in ("1.5e", "0x10", "1_0", "++1", "+-1", "--1", "1.5.2", "e5", "inf5", "+"):
+        p = tmp_path / "bad.top"
+        p.write_text("1 2\n3 4\n" + f"5 {bad}\n")
+        with pytest.raises(nat.CpetError, match="line 3"):
+            CH.read_topo_file(p)
+        if bad != "1_0":  # digit-group underscores: float() takes them, the reader does not
+            with pytest.raises(ValueError):
+                float(bad)
+    assert math.isinf(float("1e309"))
