@@ -141,20 +141,6 @@ struct Pieces {
   }
 };
 
-// Calls fn(line_start, line_len) for every line of text.
-template <typename Fn>
-void for_lines(const std::string& s, Fn fn) {
-  size_t i = 0;
-  while (i < s.size()) {
-    size_t j = s.find('\n', i);
-    if (j == std::string::npos) j = s.size();
-    size_t e = j;
-    if (e > i && s[e - 1] == '\r') --e;  // text-mode reading turns CRLF into LF
-    fn(s.data() + i, e - i);
-    i = j + 1;
-  }
-}
-
 bool is_record(const char* l, size_t n) {
   return (n >= 4 && std::memcmp(l, "ATOM", 4) == 0) || (n >= 6 && std::memcmp(l, "HETATM", 6) == 0);
 }
@@ -290,22 +276,28 @@ inline bool fast_fixed3(double v, char* out, int* len) {
 }
 
 // Python's "%.3f" / "%.18e" formatting of a float is C's, except that Python
-// spells every NaN "nan" (C prints "-nan" for a negative one).
+// spells every NaN "nan" (C prints "-nan" for a negative one).  std::to_chars
+// with an explicit precision is specified as printf's conversion with that
+// precision (correctly rounded, "inf"/"-inf", "-0.000"), at about a fifth of
+// snprintf's cost per value.
 void format_rows(const void* data, int dtype, int64_t r0, int64_t r1, int cols, int fmt,
                  std::string& out) {
   // "%.3f" of the largest double is 309 integer digits + sign + ".000"
   char buf[352];
-  const char* f = fmt == 0 ? "%.3f" : "%.18e";
+  const auto form = fmt == 0 ? std::chars_format::fixed : std::chars_format::scientific;
+  const int prec = fmt == 0 ? 3 : 18;
   for (int64_t r = r0; r < r1; ++r) {
     for (int c = 0; c < cols; ++c) {
       const double v = value_at(data, dtype, (size_t)r * cols + c);
       int n;
       if (fmt == 0 && dtype != 1 && fast_fixed3(v, buf, &n)) {
-      } else if (std::isnan(v))
-        n = std::snprintf(buf, sizeof(buf), "nan");
-      else
-        n = std::snprintf(buf, sizeof(buf), f, v);
-      out.append(buf, n < (int)sizeof(buf) ? (size_t)n : sizeof(buf) - 1);
+      } else if (std::isnan(v)) {
+        std::memcpy(buf, "nan", 3);
+        n = 3;
+      } else {
+        n = (int)(std::to_chars(buf, buf + sizeof(buf), v, form, prec).ptr - buf);
+      }
+      out.append(buf, (size_t)n);
       out.push_back(c + 1 < cols ? ' ' : '\n');
     }
   }

@@ -117,6 +117,20 @@ def test_write_rows_matches_savetxt(tmp_path, dtype, fmt):
     assert (tmp_path / "n.txt").read_bytes() == want
 
 
+@pytest.mark.parametrize("fmt", ["%.3f", "%.18e"])
+def test_write_rows_every_exponent(tmp_path, fmt):
+    """Random float64 bit patterns cover every binary exponent, subnormals and
+    three-digit decimal exponents; "%.3f" of values up to 1e308 prints the full
+    integer part."""
+    bits = np.random.default_rng(11).integers(0, 2**63, size=(4000, 5), dtype=np.uint64)
+    bits[::2] |= np.uint64(1) << np.uint64(63)
+    a = bits.view(np.float64)
+    a[np.isnan(a)] = 1.0
+    a[0, :4] = [5e-324, -2.2250738585072014e-308, 1e23, 0.0005]
+    cio.write_rows(tmp_path / "n.txt", a, fmt)
+    assert (tmp_path / "n.txt").read_bytes() == _savetxt_bytes(tmp_path / "s.txt", a, fmt)
+
+
 def test_write_rows_shapes(tmp_path):
     v = np.linspace(-3, 3, 101).astype(np.float32)
     cio.write_rows(tmp_path / "a.txt", v, "%.3f")
