@@ -376,6 +376,21 @@ int cpet_cluster_histograms(cpet_ctx *ctx, int F, const int64_t *offsets, const 
                             int nb_d, const double *edges_d, int nb_c, const double *edges_c,
                             double *hists, uint32_t flags);
 
+/* Order statistics of the columns of x (n, cols) row-major double, for the
+ * shared bins of make_histograms: its np.max / np.min and the 25th / 75th
+ * percentiles inside scipy.stats.iqr (CPET/utils/calculator.py:664-670) are
+ * all k-th smallest values of a column.  values[k] (host, n_sel doubles) =
+ * the sel_ranks[k]-th smallest (0-based) value of column sel_cols[k], an
+ * element of that column, exact; NaNs order after every number, as in
+ * numpy's sort, and -0.0 is returned as +0.0.  nan_counts (host, cols int64,
+ * or NULL) = the NaNs of each column, for numpy's rule that a NaN anywhere
+ * makes max, min and percentile NaN.  1 <= cols <= 8, 1 <= n_sel <= 16,
+ * 1 <= n < 2^32; x is a device pointer with CPET_F_DEVICE; the call waits
+ * for the stream. */
+int cpet_order_stats(cpet_ctx *ctx, int64_t n, int cols, const double *x, int n_sel,
+                     const int32_t *sel_cols, const int64_t *sel_ranks, double *values,
+                     int64_t *nan_counts, uint32_t flags);
+
 /* out (F,F) double: out[i][j] = sum over bins with h_i + h_j != 0 of
  * (h_i - h_j)^2 / (h_i + h_j), halved; zero diagonal.  Replaces
  * distance_numpy + construct_distance_matrix (CPET/utils/calculator.py:

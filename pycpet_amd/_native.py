@@ -94,6 +94,7 @@ SIGNATURES = {
     "cpet_ctx_refined_points": [c_void_p, P],
     "cpet_cluster_histograms": [c_void_p, c_int, P, P, c_int, P, c_int, P, P, c_uint32],
     "cpet_chi2_matrix": [c_void_p, c_int, c_int, P, P, c_uint32],
+    "cpet_order_stats": [c_void_p, ctypes.c_int64, c_int, P, c_int, P, P, P, P, c_uint32],
     "cpet_read_rows": [ctypes.c_char_p, c_int, P, ctypes.c_int64, P],
     "cpet_pdb_scan": [ctypes.c_char_p, P],
     "cpet_pdb_parse": [ctypes.c_char_p, c_int, c_int, P, P, P, P, P, P, P],

@@ -9,9 +9,12 @@ and return values.
 The host side keeps the reference's bin rule: global min / max over every
 frame's features, bin width 2 * IQR / n^(1/3) (scipy.stats.iqr, n the row
 count, or its mean when frames differ in length), int((max - min) / width)
-bins, and the edges numpy.histogram2d builds from (bins, range).  The .top
-files are read by cpet_read_rows; the histograms (cpet_cluster_histograms)
-and the distance matrix (cpet_chi2_matrix) run on the GPU.
+bins, and the edges numpy.histogram2d builds from (bins, range).  The min,
+max and the two quartiles come from exact order statistics selected on the
+GPU (cpet_order_stats), combined on the host with numpy.percentile's linear
+interpolation step for step.  The .top files are read by cpet_read_rows; the
+histograms (cpet_cluster_histograms) and the distance matrix
+(cpet_chi2_matrix) run on the GPU.
 """
 from __future__ import annotations
 
@@ -48,23 +51,100 @@ def _outer_edges(lo, hi):
     return lo, hi
 
 
-def histogram_edges(features: Sequence[np.ndarray]) -> Tuple[np.ndarray, np.ndarray]:
-    """The shared (distance, curvature) bin edges of make_histograms."""
-    from scipy.stats import iqr
+# scipy.stats.iqr's default rng=(25, 75), divided by 100 as numpy.percentile
+# divides it (both exact in binary)
+_IQR_Q = (0.25, 0.75)
 
+
+def _binning_count(features):
+    """The n of the bin width: the common row count, or (with the
+    reference's warning) the mean one (CPET/utils/calculator.py:654-662)."""
     lens = np.array([len(f) for f in features], dtype=int)
     if lens.size and not np.all(lens == lens[0]):
         warnings.warn(
             f"Topologies provided are of different sizes, using the mean value of "
             f"{np.mean(lens)} to represent binning for {lens}")
-        n = np.mean(lens)
-    else:
-        n = lens[0]
-    allf = np.concatenate([np.asarray(f, np.float64).reshape(-1, 2) for f in features])
-    d, c = allf[:, 0], allf[:, 1]
-    dmax, cmax, dmin, cmin = np.max(d), np.max(c), np.min(d), np.min(c)
-    d_res = 2 * iqr(d) / (n ** (1 / 3))
-    c_res = 2 * iqr(c) / (n ** (1 / 3))
+        return np.mean(lens)
+    return lens[0]
+
+
+def _flatten(features):
+    feats = [np.ascontiguousarray(np.asarray(f, np.float64).reshape(-1, 2)) for f in features]
+    off = np.zeros(len(feats) + 1, np.int64)
+    off[1:] = np.cumsum([len(f) for f in feats])
+    allf = np.concatenate(feats) if off[-1] else np.zeros((0, 2))
+    return feats, off, allf
+
+
+def _lerp_points(count, q):
+    """numpy.percentile's 'linear' method (numpy/lib/_function_base_impl.py:
+    _quantile, _get_indexes, _get_gamma) for `count` values: the two ranks
+    it reads and the weight gamma.  At or past the last index both ranks are
+    the last one and gamma = virtual index + 1, as numpy computes it."""
+    v = np.float64(count - 1) * np.float64(q)
+    if v >= count - 1:
+        return count - 1, count - 1, v - np.float64(-1.0)
+    lo = np.floor(v)
+    return int(lo), int(lo) + 1, v - lo
+
+
+def _percentile_from_order(count, q, kth):
+    """numpy.percentile(x, 100 q) (linear) from kth(r), the r-th smallest
+    value of x: numpy's _lerp, a + (b - a) t, or b - (b - a)(1 - t) for
+    t >= 0.5, in float64."""
+    i, j, g = _lerp_points(count, q)
+    a, b = np.float64(kth(i)), np.float64(kth(j))
+    diff = b - a
+    return b - diff * (np.float64(1.0) - g) if g >= 0.5 else a + diff * g
+
+
+def _stats_from_order(count, kth, nan_count=0):
+    """(min, max, iqr) of one column as np.min, np.max and scipy.stats.iqr
+    give them (CPET/utils/calculator.py:664-670): from its order statistics
+    kth(r), with numpy's rule that any NaN makes all three NaN."""
+    if nan_count:
+        nan = np.float64(np.nan)
+        return nan, nan, nan
+    p25, p75 = (_percentile_from_order(count, q, kth) for q in _IQR_Q)
+    return np.float64(kth(0)), np.float64(kth(count - 1)), np.subtract(p75, p25)
+
+
+def _order_stat_ranks(count):
+    ranks = {0, count - 1}
+    for q in _IQR_Q:
+        i, j, _ = _lerp_points(count, q)
+        ranks |= {i, j}
+    return sorted(ranks)
+
+
+def _column_stats(allf, ctx):
+    """[(min, max, iqr)] of the two feature columns, from order statistics
+    selected on the GPU (cpet_order_stats: exact elements of the column)."""
+    P = allf.shape[0]
+    ranks = _order_stat_ranks(P)
+    plan = [(c, r) for c in range(2) for r in ranks]
+    cols = np.array([c for c, _ in plan], np.int32)
+    rk = np.array([r for _, r in plan], np.int64)
+    vals = np.empty(len(plan), np.float64)
+    nans = np.zeros(2, np.int64)
+    x = np.ascontiguousarray(allf, np.float64)
+    nat.check(ctx.lib.cpet_order_stats(ctx.handle, P, 2, x.ctypes.data, len(plan),
+                                       cols.ctypes.data, rk.ctypes.data, vals.ctypes.data,
+                                       nans.ctypes.data, 0), ctx.lib)
+    at = {key: k for k, key in enumerate(plan)}
+    return [_stats_from_order(P, lambda r, c=c: vals[at[(c, r)]], int(nans[c]))
+            for c in range(2)]
+
+
+def _edges_from_stats(stats, n):
+    """Shared edges from [(min, max, iqr)] of the distance and curvature
+    columns and the binning count n (CPET/utils/calculator.py:669-670,
+    684-685, 702-707):
+    width 2 iqr / n^(1/3), int((max - min) / width) bins, numpy.histogram2d's
+    edges over [min, max]."""
+    (dmin, dmax, d_iqr), (cmin, cmax, c_iqr) = stats
+    d_res = 2 * d_iqr / (n ** (1 / 3))
+    c_res = 2 * c_iqr / (n ** (1 / 3))
     nd = int((dmax - dmin) / d_res)
     nc = int((cmax - cmin) / c_res)
     for k, nb in ((0, nd), (1, nc)):
@@ -75,17 +155,31 @@ def histogram_edges(features: Sequence[np.ndarray]) -> Tuple[np.ndarray, np.ndar
     return ed, ec
 
 
+def _edges(allf, n, ctx):
+    if allf.shape[0] == 0:
+        np.max(allf[:, 0])  # numpy's error for an empty reduction, as the reference
+    return _edges_from_stats(_column_stats(allf, ctx), n)
+
+
+def histogram_edges(features: Sequence[np.ndarray],
+                    ctx: Optional[nat.Context] = None) -> Tuple[np.ndarray, np.ndarray]:
+    """The shared (distance, curvature) bin edges of make_histograms."""
+    ctx = ctx or nat.default_context()
+    n = _binning_count(features)
+    return _edges(_flatten(features)[2], n, ctx)
+
+
 def histograms_from_features(features: Sequence[np.ndarray], edges=None,
                              ctx: Optional[nat.Context] = None) -> np.ndarray:
     """(F, nd*nc) normalised histograms of per-frame (L_f, 2) features."""
     ctx = ctx or nat.default_context()
-    ed, ec = edges if edges is not None else histogram_edges(features)
+    n = _binning_count(features) if edges is None else None
+    feats, off, allf = _flatten(features)
+    ed, ec = edges if edges is not None else _edges(allf, n, ctx)
     ed = np.ascontiguousarray(ed, np.float64)
     ec = np.ascontiguousarray(ec, np.float64)
-    feats = [np.ascontiguousarray(np.asarray(f, np.float64).reshape(-1, 2)) for f in features]
-    off = np.zeros(len(feats) + 1, np.int64)
-    off[1:] = np.cumsum([len(f) for f in feats])
-    allf = np.concatenate(feats) if off[-1] else np.zeros((1, 2))
+    if not off[-1]:
+        allf = np.zeros((1, 2))
     nd, nc = len(ed) - 1, len(ec) - 1
     out = np.empty((len(feats), nd * nc), np.float64)
     nat.check(ctx.lib.cpet_cluster_histograms(

@@ -87,3 +87,117 @@ def test_large_matrix_properties(gpu_ctx):
     for i, j in rng.integers(0, 1000, (40, 2)):
         if i != j:
             assert abs(m[i, j] - CH.distance_numpy(h[i], h[j])) <= 1e-12 * max(m[i, j], 1e-300)
+
+
+def _order_stats(ctx, x, sel):
+    """cpet_order_stats on the (n, cols) array x for [(col, rank)]."""
+    from pycpet_amd import _native as nat
+
+    x = np.ascontiguousarray(x, np.float64)
+    n, cols = x.shape
+    sc = np.array([c for c, _ in sel], np.int32)
+    sr = np.array([r for _, r in sel], np.int64)
+    vals = np.empty(len(sel))
+    nans = np.zeros(cols, np.int64)
+    nat.check(ctx.lib.cpet_order_stats(ctx.handle, n, cols, x.ctypes.data, len(sel),
+                                       sc.ctypes.data, sr.ctypes.data, vals.ctypes.data,
+                                       nans.ctypes.data, 0), ctx.lib)
+    return vals, nans
+
+
+def _sorted_cols(x):
+    """numpy's order (NaNs last) with -0.0 read as +0.0."""
+    s = np.sort(x, axis=0)
+    s[s == 0.0] = 0.0
+    return s
+
+
+def test_order_stats_exact_on_awkward_columns(gpu_ctx):
+    """Every requested k-th smallest value is the element numpy's sort puts at
+    k: ties, signed zeros, subnormals, infinities, NaNs (counted, ordered
+    last), one to eight columns, 1 to 16 selections, sizes 1 to 300k."""
+    rng = np.random.default_rng(4)
+    cases = []
+    for n in (1, 2, 3, 7, 64, 255, 256, 257, 1000, 300_001):
+        for cols in (1, 2, 3, 8):
+            x = rng.standard_normal((n, cols)) * 10.0 ** rng.integers(-300, 300, (n, cols))
+            kind = rng.integers(0, 4)
+            if kind == 1:
+                x = rng.integers(-3, 4, (n, cols)).astype(np.float64)  # heavy ties
+            elif kind == 2:
+                pick = rng.random((n, cols))
+                x[pick < 0.1] = -0.0
+                x[(pick >= 0.1) & (pick < 0.2)] = 0.0
+                x[(pick >= 0.2) & (pick < 0.25)] = 5e-324
+                x[(pick >= 0.25) & (pick < 0.28)] = -np.inf
+                x[(pick >= 0.28) & (pick < 0.3)] = np.inf
+            elif kind == 3:
+                x[rng.random((n, cols)) < 0.05] = np.nan
+            cases.append(x)
+    for x in cases:
+        n, cols = x.shape
+        m = int(rng.integers(1, 17))
+        sel = [(int(rng.integers(0, cols)), int(r)) for r in rng.integers(0, n, m)]
+        sel[0] = (sel[0][0], 0)
+        sel[-1] = (sel[-1][0], n - 1)
+        print(f"order stats n={n} cols={cols} sel={len(sel)}", flush=True)
+        vals, nans = _order_stats(gpu_ctx, x, sel)
+        srt = _sorted_cols(x)
+        want = np.array([srt[r, c] for c, r in sel])
+        np.testing.assert_array_equal(vals, want)
+        assert not np.any(np.signbit(vals[vals == 0.0]))
+        np.testing.assert_array_equal(nans, np.isnan(x).sum(0))
+
+
+def test_order_stats_bench_size(gpu_ctx):
+    """The cluster workload's 256 frames x 20,000 rows in one call: the six
+    ranks per column that min, max and the quartiles read."""
+    from pycpet_amd import clusterhist as CH
+
+    rng = np.random.default_rng(8)
+    x = np.stack([rng.gamma(2.0, 0.3, 5_120_000), rng.lognormal(0.0, 0.5, 5_120_000)], 1)
+    x = x.astype(np.float32).astype(np.float64)  # float32 .top values: many ties
+    ranks = CH._order_stat_ranks(len(x))
+    sel = [(c, r) for c in range(2) for r in ranks]
+    vals, nans = _order_stats(gpu_ctx, x, sel)
+    for c in range(2):
+        part = np.partition(x[:, c], ranks)
+        for k, r in enumerate(ranks):
+            assert vals[c * len(ranks) + k] == part[r]
+    assert not nans.any()
+
+
+def test_histogram_edges_on_gpu_match_oracle(gpu_ctx, oracle_mod):
+    """histogram_edges (GPU order statistics + numpy's interpolation on the
+    host) equals the oracle's scipy.stats.iqr / np.min / np.max edges bit for
+    bit, for equal and ragged frames, and fails as the reference does on a
+    NaN feature (int() of a NaN bin count)."""
+    from pycpet_amd import clusterhist as CH
+
+    for seed, ragged in ((0, False), (1, True), (2, False)):
+        feats = _frames(20 + seed, 40, 3000, ragged)
+        if seed == 2:  # float32 rows with exact duplicates of the quartiles
+            feats[5][:100] = feats[0][:100]
+        _, (xe, ye) = oracle_mod.cluster_histograms(feats)
+        if ragged:
+            with pytest.warns(UserWarning, match="different sizes"):
+                ed, ec = CH.histogram_edges(feats, ctx=gpu_ctx)
+        else:
+            ed, ec = CH.histogram_edges(feats, ctx=gpu_ctx)
+        np.testing.assert_array_equal(ed, xe)
+        np.testing.assert_array_equal(ec, ye)
+    feats = _frames(30, 4, 500)
+    feats[2][7, 1] = np.nan
+    with pytest.raises(ValueError, match="NaN"):
+        CH.histogram_edges(feats, ctx=gpu_ctx)
+
+
+def test_order_stats_rejects_bad_selections(gpu_ctx):
+    from pycpet_amd._native import CpetError
+
+    x = np.zeros((10, 2))
+    for sel in ([(0, 10)], [(2, 0)], [(0, -1)], [(0, 0)] * 17):
+        with pytest.raises(CpetError, match="cpet_order_stats"):
+            _order_stats(gpu_ctx, x, sel)
+    with pytest.raises(CpetError, match="cpet_order_stats"):
+        _order_stats(gpu_ctx, np.zeros((4, 9)), [(0, 0)])

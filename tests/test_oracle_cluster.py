@@ -44,21 +44,81 @@ def test_oracle_chi2_closed_forms(oracle_mod):
     np.testing.assert_array_equal(m, m.T)
 
 
-def test_bin_edges_match_numpy_histogram2d(oracle_mod):
-    from pycpet_amd.clusterhist import histogram_edges
+def _np_kth(col):
+    """r -> the r-th smallest of col, NaNs last (numpy's sort order)."""
+    srt = np.sort(np.asarray(col, np.float64))
+    return lambda r: srt[r]
 
+
+def _edges_numpy_stats(feats):
+    """histogram_edges' host steps with numpy's own order statistics in place
+    of the GPU selection."""
+    from pycpet_amd import clusterhist as CH
+
+    n = CH._binning_count(feats)
+    allf = CH._flatten(feats)[2]
+    P = allf.shape[0]
+    stats = [CH._stats_from_order(P, _np_kth(allf[:, c]), int(np.isnan(allf[:, c]).sum()))
+             for c in range(2)]
+    return CH._edges_from_stats(stats, n)
+
+
+def test_bin_edges_match_numpy_histogram2d(oracle_mod):
     for seed in range(3):
         feats = _frames(seed, F=4, L=200 + 50 * seed)
         if seed:  # ragged frames: binning uses the mean row count
             feats = [f[:len(f) - 17 * k] for k, f in enumerate(feats)]
         if seed == 0:
-            ed, ec = histogram_edges(feats)
+            ed, ec = _edges_numpy_stats(feats)
         else:
             with pytest.warns(UserWarning, match="different sizes"):
-                ed, ec = histogram_edges(feats)
+                ed, ec = _edges_numpy_stats(feats)
         _, (xe, ye) = oracle_mod.cluster_histograms(feats)
         np.testing.assert_array_equal(ed, xe)
         np.testing.assert_array_equal(ec, ye)
+
+
+def _adversarial_columns():
+    rng = np.random.default_rng(21)
+    cols = [np.array([v]) for v in (2.5, -0.0, np.inf, -np.inf)]
+    for n in (2, 3, 4, 5, 6, 7, 8, 9, 13):
+        cols.append(rng.standard_normal(n))
+        cols.append(rng.integers(-2, 3, n).astype(np.float64))  # ties
+    cols.append(np.array([1.0, np.inf]))
+    cols.append(np.array([-np.inf, 0.0, 1.0, np.inf]))
+    cols.append(np.array([0.0, -0.0, 0.0, -0.0, 1e-310, -5e-324]))
+    cols.append(np.array([1.0, np.nan, 2.0]))
+    cols.append(np.array([1e308, -1e308, 1.7976931348623157e308, 3.0]))
+    cols.append(rng.standard_normal(1001) * 1e-300)
+    cols.append(np.repeat(rng.standard_normal(7), 143))
+    return cols
+
+
+def test_column_stats_match_numpy_and_scipy():
+    """min, max and the IQR from order statistics equal np.min, np.max and
+    scipy.stats.iqr bit for bit, with numpy's NaN and infinity results
+    (inf - inf in the interpolation gives NaN there too)."""
+    from scipy.stats import iqr
+
+    from pycpet_amd import clusterhist as CH
+
+    for col in _adversarial_columns():
+        with np.errstate(invalid="ignore"):
+            got = CH._stats_from_order(len(col), _np_kth(col), int(np.isnan(col).sum()))
+            want = (np.min(col), np.max(col), iqr(col))
+        np.testing.assert_array_equal(np.array(got), np.array(want), err_msg=repr(col))
+
+
+def test_order_stat_ranks_cover_the_interpolation():
+    from pycpet_amd import clusterhist as CH
+
+    for n in (1, 2, 3, 4, 5, 100, 5_120_000):
+        ranks = CH._order_stat_ranks(n)
+        assert ranks[0] == 0 and ranks[-1] == n - 1 and len(ranks) <= 6
+        for q in CH._IQR_Q:
+            i, j, g = CH._lerp_points(n, q)
+            assert i in ranks and j in ranks and 0 <= i <= j < n
+            assert g == (n - 1) * q - i or n == 1
 
 
 def test_topo_reader_matches_loadtxt(tmp_path):
