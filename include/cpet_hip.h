@@ -45,6 +45,8 @@ extern "C" {
 #define CPET_F_TOPO_RK4 0x20u  /* cpet_topo: classical RK4 steps on E/|E| for the walk and the
                                   curvature steps; no counterpart in the reference, which
                                   integrates with Euler (math_module.c:489-503) */
+#define CPET_F_COUNTS 0x40u  /* cpet_cluster_histograms: the raw bin counts (as doubles)
+                                instead of count / total */
 /* precision of the field kernels, bits 8..10 */
 #define CPET_F_PREC_AUTO (0u << 8)  /* E: FP32 tiled, points with strong
                                        cancellation recomputed as MIXED;
@@ -369,7 +371,9 @@ int cpet_set_tuning(cpet_ctx *ctx, int kind, int a, int b);
  * counts over the shared edges with numpy.histogramdd's rule (searchsorted
  * right; a value equal to the last edge in the last bin; others outside are
  * dropped), then divided by the frame's own in-range count (0/0 = NaN, as
- * numpy).  hists (F, nb_d*nb_c) double, row-major [bd][bc].  offsets (F+1,
+ * numpy), or the counts themselves with CPET_F_COUNTS (make_histograms_mem,
+ * CPET/utils/calculator.py:721-845, normalises numpy's density on the host).
+ * hists (F, nb_d*nb_c) double, row-major [bd][bc].  offsets (F+1,
  * int64, starting at 0) and the edges (nb_d+1, nb_c+1 doubles, increasing)
  * are host arrays; feats and hists are device pointers with CPET_F_DEVICE. */
 int cpet_cluster_histograms(cpet_ctx *ctx, int F, const int64_t *offsets, const double *feats,

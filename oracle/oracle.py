@@ -416,6 +416,61 @@ def cluster_histograms(features):
     return np.array(out), edges
 
 
+def cluster_histograms_mem(features):
+    """make_histograms_mem (CPET/utils/calculator.py:721-845) on in-memory
+    per-frame (L, 2) features: Python min / max over the rows, scipy iqr,
+    matplotlib's hist2d with density=True, then each value divided by the
+    row-by-row sum, as the reference does.  Returns (F, B) float64.  Pinned
+    by tests/test_oracle_cluster.py (brute-force counts / total, the edges
+    of cluster_histograms)."""
+    import matplotlib
+
+    matplotlib.use("Agg")
+    import matplotlib.pyplot as plt
+    from scipy.stats import iqr
+
+    dist_list, curv_list, lens = [], [], []
+    for f in features:
+        f = np.asarray(f, float)
+        dist_list.extend(f[:, 0].tolist())
+        curv_list.extend(f[:, 1].tolist())
+        lens.append(len(f))
+    n = lens[0] if all(x == lens[0] for x in lens) else np.mean(lens)
+    dmax, cmax, dmin, cmin = max(dist_list), max(curv_list), min(dist_list), min(curv_list)
+    dres = 2 * iqr(dist_list) / (n ** (1 / 3))
+    cres = 2 * iqr(curv_list) / (n ** (1 / 3))
+    nd = int((dmax - dmin) / dres)
+    nc = int((cmax - cmin) / cres)
+    out = []
+    fig = plt.figure()
+    try:
+        for f in features:
+            f = np.asarray(f, float)
+            a, _, _, _ = plt.hist2d(f[:, 0].tolist(), f[:, 1].tolist(), bins=(nd, nc),
+                                    range=[[dmin, dmax], [cmin, cmax]], density=True)
+            norm = 0
+            for row in a:
+                for m in row:
+                    norm += m
+            out.append(np.array([[m / norm for m in row] for row in a]).flatten())
+    finally:
+        plt.close(fig)
+    return np.array(out)
+
+
+def chi2_matrix_mem(hists):
+    """construct_distance_matrix_mem (CPET/utils/calculator.py:981-1000) on
+    in-memory histograms: distance_numpy over every pair i < j, mirrored,
+    zero diagonal."""
+    n = len(hists)
+    m = np.zeros((n, n), dtype=np.float64)
+    for i in range(n):
+        for j in range(i + 1, n):
+            m[i][j] = chi2_distance(np.asarray(hists[i]), np.asarray(hists[j]))
+            m[j][i] = m[i][j]
+    return m
+
+
 def chi2_distance(h1, h2):
     """distance_numpy, CPET/utils/calculator.py:975-978."""
     a = (h1 - h2) ** 2

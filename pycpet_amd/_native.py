@@ -30,6 +30,7 @@ F_OUT_F64 = 0x2
 F_NO_SYNC = 0x4
 F_DIPOLE_PHYSICAL = 0x10
 F_TOPO_RK4 = 0x20
+F_COUNTS = 0x40
 PREC = {"auto": 0 << 8, "naive": 1 << 8, "kahan": 2 << 8, "f64": 3 << 8,
         "tiled": 4 << 8, "mixed": 5 << 8}
 

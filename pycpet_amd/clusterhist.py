@@ -3,8 +3,10 @@ chi-square distance matrix, with the reference's function names, arguments
 and return values.
 
   make_histograms(topo_files)         CPET/utils/calculator.py:596-718
+  make_histograms_mem(files, outdir)  CPET/utils/calculator.py:721-845
   distance_numpy(hist1, hist2)        CPET/utils/calculator.py:975-978
-  construct_distance_matrix(hists)    CPET/utils/calculator.py:1000-1015
+  construct_distance_matrix_mem(files) CPET/utils/calculator.py:981-1000
+  construct_distance_matrix(hists)    CPET/utils/calculator.py:1003-1015
 
 The host side keeps the reference's bin rule: global min / max over every
 frame's features, bin width 2 * IQR / n^(1/3) (scipy.stats.iqr, n the row
@@ -188,6 +190,88 @@ def histograms_from_features(features: Sequence[np.ndarray], edges=None,
     return out
 
 
+def _histogram_counts(feats, off, allf, ed, ec, ctx) -> np.ndarray:
+    """(F, nd, nc) float64 bin counts over the shared edges (cpet_cluster_histograms
+    with CPET_F_COUNTS)."""
+    ed = np.ascontiguousarray(ed, np.float64)
+    ec = np.ascontiguousarray(ec, np.float64)
+    if not off[-1]:
+        allf = np.zeros((1, 2))
+    nd, nc = len(ed) - 1, len(ec) - 1
+    out = np.empty((len(feats), nd, nc), np.float64)
+    nat.check(ctx.lib.cpet_cluster_histograms(
+        ctx.handle, len(feats), off.ctypes.data, allf.ctypes.data, nd, ed.ctypes.data, nc,
+        ec.ctypes.data, out.ctypes.data, nat.F_COUNTS), ctx.lib)
+    return out
+
+
+def _read_topo_any_sep(path) -> np.ndarray:
+    """make_histograms_mem's reader (CPET/utils/calculator.py:735-748): a
+    line that holds a comma is split on commas, any other on whitespace; the
+    first two fields of every line not starting with "#".  Files without a
+    comma go through the native reader, which follows the whitespace rule."""
+    with open(path, "rb") as fh:
+        has_comma = b"," in fh.read()
+    if not has_comma:
+        return read_topo_file(path)
+    rows = []
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("#"):
+                continue
+            line = line.strip()
+            parts = line.split(",") if "," in line else line.split()
+            rows.append((float(parts[0]), float(parts[1])))
+    return np.array(rows, np.float64).reshape(-1, 2)
+
+
+def _density_normalised(counts, ed, ec) -> np.ndarray:
+    """make_histograms_mem's values from one frame's (nd, nc) counts:
+    numpy.histogramdd's density=True steps (numpy/lib/_histograms_impl.py:
+    count / dx_i / dy_j / total, via plt.hist2d), then the reference divides
+    every value by their sum taken row by row in order
+    (CPET/utils/calculator.py:813-834)."""
+    hist = np.asarray(counts, np.float64)
+    s = hist.sum()
+    with np.errstate(invalid="ignore", divide="ignore"):
+        hist = hist / np.diff(ed).reshape(-1, 1)
+        hist = hist / np.diff(ec).reshape(1, -1)
+        hist /= s
+        flat = hist.ravel()
+        norm = np.cumsum(flat)[-1] if flat.size else 0.0  # the sequential sum
+        return flat / norm
+
+
+def make_histograms_mem(topo_files, output_dir, plot=False,
+                        ctx: Optional[nat.Context] = None):
+    """The reference's file-by-file variant (CPET/utils/calculator.py:
+    721-845): the same shared bins as make_histograms, each frame's
+    histogram saved as <output_dir>/<file name without its 4-character
+    extension>_hist.npy (the density values, normalised to sum 1), and the
+    list of those paths returned.  The counts run on the GPU."""
+    import os
+
+    ctx = ctx or nat.default_context()
+    feats = [_read_topo_any_sep(f) for f in topo_files]
+    lens = [len(f) for f in feats]
+    if lens and not all(x == lens[0] for x in lens):
+        warnings.warn(
+            f"Topologies provided are of different sizes, using the mean value of "
+            f"{np.mean(lens)} to represent binning for {lens}")
+        n = np.mean(lens)
+    else:
+        n = lens[0]
+    feats, off, allf = _flatten(feats)
+    ed, ec = _edges(allf, n, ctx)
+    counts = _histogram_counts(feats, off, allf, ed, ec, ctx)
+    hist_list = []
+    for f, c in zip(topo_files, counts):
+        outfile = os.path.join(output_dir, f"{str(f).split('/')[-1][:-4]}_hist.npy")
+        np.save(outfile, _density_normalised(c, ed, ec))
+        hist_list.append(outfile)
+    return hist_list
+
+
 def make_histograms(topo_files, plot=False, ctx: Optional[nat.Context] = None) -> np.ndarray:
     """Normalised, flattened 2-D histograms of every .top file's features on
     shared bins; one row per file (CPET/utils/calculator.py:596-718)."""
@@ -201,6 +285,17 @@ def distance_numpy(hist1, hist2):
     a = (hist1 - hist2) ** 2
     b = hist1 + hist2
     return np.sum(np.divide(a, b, out=np.zeros_like(a), where=b != 0)) / 2.0
+
+
+def construct_distance_matrix_mem(hist_file_list, ctx: Optional[nat.Context] = None) -> np.ndarray:
+    """(F, F) chi-square distances of histograms saved as .npy files
+    (CPET/utils/calculator.py:981-1000): the reference loads two files per
+    pair; here every file is loaded once and the matrix runs on the GPU (F
+    histograms held at once: F x bins doubles)."""
+    hists = [np.load(p) for p in hist_file_list]
+    if not hists:
+        return np.zeros((0, 0), np.float64)
+    return construct_distance_matrix(np.stack([np.ravel(h) for h in hists]), ctx=ctx)
 
 
 def construct_distance_matrix(histograms, ctx: Optional[nat.Context] = None) -> np.ndarray:

@@ -41,10 +41,18 @@ __global__ __launch_bounds__(256) void frame_hist_kernel(
 }
 
 // One workgroup per frame: total = sum of the frame's counts (exact in
-// 64-bit integers), then hist = count / total (0 / 0 = NaN as in numpy).
+// 64-bit integers), then hist = count / total (0 / 0 = NaN as in numpy), or
+// the counts themselves when !normalise.
 __global__ __launch_bounds__(256) void frame_norm_kernel(int B,
                                                           const unsigned int* __restrict__ counts,
-                                                          double* __restrict__ hist) {
+                                                          double* __restrict__ hist,
+                                                          int normalise) {
+  if (!normalise) {
+    const unsigned int* c = counts + (size_t)blockIdx.x * B;
+    double* h = hist + (size_t)blockIdx.x * B;
+    for (int b = threadIdx.x; b < B; b += blockDim.x) h[b] = (double)c[b];
+    return;
+  }
   __shared__ unsigned long long part[256];
   const int f = blockIdx.x;
   const unsigned int* c = counts + (size_t)f * B;
@@ -300,7 +308,8 @@ int cpet_cluster_histograms(cpet_ctx* ctx, int F, const int64_t* offsets, const 
     if ((rc = ctx_buf(ctx, kSlotCluster2, (size_t)nc * sizeof(double), &ob))) return rc;
     dh = (double*)ob;
   }
-  hipLaunchKernelGGL(frame_norm_kernel, dim3(F), dim3(256), 0, st, (int)B, counts, dh);
+  hipLaunchKernelGGL(frame_norm_kernel, dim3(F), dim3(256), 0, st, (int)B, counts, dh,
+                     (flags & CPET_F_COUNTS) ? 0 : 1);
   CPET_HIP_TRY(hipGetLastError());
   if (!dev) CPET_HIP_TRY(hipMemcpyAsync(hists, dh, (size_t)nc * sizeof(double), hipMemcpyDeviceToHost, st));
   if (!dev || !(flags & CPET_F_NO_SYNC)) CPET_HIP_TRY(hipStreamSynchronize(st));

@@ -201,3 +201,33 @@ def test_order_stats_rejects_bad_selections(gpu_ctx):
             _order_stats(gpu_ctx, x, sel)
     with pytest.raises(CpetError, match="cpet_order_stats"):
         _order_stats(gpu_ctx, np.zeros((4, 9)), [(0, 0)])
+
+
+def test_make_histograms_mem_files(gpu_ctx, oracle_mod, tmp_path):
+    """The file-by-file variant: per-frame .npy histograms bit-equal to the
+    oracle's (hist2d density, then the reference's normalisation), frames
+    written with commas or whitespace, and the distance matrix from those
+    files against the pairwise oracle."""
+    from pycpet_amd import clusterhist as CH
+    from pycpet_amd.io import write_rows
+
+    feats = _frames(40, 7, 900, ragged=True)
+    files = []
+    for k, f in enumerate(feats):
+        p = tmp_path / f"frame{k}.top"
+        if k % 3 == 1:
+            p.write_text("# comma separated\n" + "".join(f"{a!r},{b!r}\n" for a, b in f.tolist()))
+        else:
+            write_rows(p, f, "%.18e")
+        files.append(str(p))
+    out = tmp_path / "hists"
+    out.mkdir()
+    with pytest.warns(UserWarning, match="different sizes"):
+        paths = CH.make_histograms_mem(files, str(out), ctx=gpu_ctx)
+    assert paths == [str(out / f"frame{k}_hist.npy") for k in range(len(files))]
+    want = oracle_mod.cluster_histograms_mem(feats)
+    got = np.stack([np.load(p) for p in paths])
+    np.testing.assert_array_equal(got, want)
+    m = CH.construct_distance_matrix_mem(paths, ctx=gpu_ctx)
+    np.testing.assert_allclose(m, oracle_mod.chi2_matrix_mem(list(want)), rtol=1e-12, atol=1e-15)
+    assert np.all(np.diag(m) == 0.0)

@@ -138,3 +138,47 @@ def test_topo_reader_matches_loadtxt(tmp_path):
     from pycpet_amd._native import CpetError
     with pytest.raises(CpetError, match="line 2"):
         read_topo_file(p)
+
+
+def test_oracle_mem_histograms_brute_force(oracle_mod):
+    """make_histograms_mem's oracle (matplotlib hist2d with density=True,
+    then the reference's row-by-row normalisation) against brute-force
+    counts: the values are count / total to rounding, sum to 1, and use the
+    same edges as make_histograms.  The host step that turns counts into
+    those values (clusterhist._density_normalised) reproduces them bit for
+    bit from the brute-force counts."""
+    from pycpet_amd import clusterhist as CH
+
+    for seed in range(3):
+        feats = _frames(10 + seed, F=4, L=250)
+        if seed == 2:
+            feats = [f[:len(f) - 23 * k] for k, f in enumerate(feats)]
+        mem = oracle_mod.cluster_histograms_mem(feats)
+        _, (xe, ye) = oracle_mod.cluster_histograms(feats)
+        for f, h in zip(feats, mem):
+            cnt = _brute_counts(f, xe, ye)
+            np.testing.assert_allclose(h, (cnt / cnt.sum()).ravel(), rtol=1e-14, atol=1e-17)
+            assert abs(h.sum() - 1.0) < 1e-12
+            np.testing.assert_array_equal(CH._density_normalised(cnt, xe, ye), h)
+
+
+def test_oracle_chi2_mem_equals_pairwise_path(oracle_mod):
+    rng = np.random.default_rng(2)
+    h = rng.random((9, 40)) ** 3
+    h[rng.random(h.shape) < 0.3] = 0.0
+    h /= h.sum(1, keepdims=True)
+    m = oracle_mod.chi2_matrix_mem(list(h))
+    np.testing.assert_allclose(m, oracle_mod.chi2_matrix(h), rtol=1e-14, atol=1e-17)
+    assert np.all(np.diag(m) == 0.0)
+
+
+def test_mem_reader_commas_and_whitespace(tmp_path):
+    from pycpet_amd import clusterhist as CH
+
+    p = tmp_path / "c.top"
+    p.write_text("# header\n1.5,2.25\n 3.0 , 4.0 \n5e-1\t6\n#x\n7,8,9\n")
+    np.testing.assert_array_equal(CH._read_topo_any_sep(p),
+                                  [[1.5, 2.25], [3.0, 4.0], [0.5, 6.0], [7.0, 8.0]])
+    q = tmp_path / "w.top"
+    q.write_text("# header\n1.5 2.25\n3 4 99\n")
+    np.testing.assert_array_equal(CH._read_topo_any_sep(q), [[1.5, 2.25], [3.0, 4.0]])
