@@ -260,3 +260,18 @@ class calculator:
         """Same kernels as compute_topo_complete_c_shared (the reference's
         torch path computes the same quantities, CPET/utils/gpu.py)."""
         return self._topo()
+
+    def compute_topo_complete_c_shared_dipole(self):
+        """(L,2) float32 [distance, mean curvature] per start in the field of
+        point dipoles self.x (N,3) with moments self.mu (N,3), read at call
+        time like the reference (CPET/source/calculator.py:714-749, called by
+        scripts/field_topology_dipole.py); the reference's dipole pair term
+        (math_module.c:335-371, 593-659)."""
+        t0 = time.time()
+        r = ops.topo_dipole(self.random_start_points, np.asarray(self.random_max_samples),
+                            self.step_size, self.dimensions, self.x, self.mu)
+        self.topo_result = r
+        self.hist = r.feats.astype(np.float32)
+        self.log.debug("Time taken for %d calculations with N_dipoles = %d: %.3f s",
+                       len(self.random_start_points), len(self.mu), time.time() - t0)
+        return self.hist

@@ -70,3 +70,72 @@ def test_dropin_thread_operation_dipole(gpu_ctx):
     batch = ops.topo_dipole(g["starts"], g["n_iter"], float(g["h"]), g["dims"], g["x"], g["mu"],
                             ctx=gpu_ctx).feats.astype(np.float32)
     np.testing.assert_array_equal(got, batch)
+
+
+def test_calculator_dipole_method_as_the_script_builds_it(gpu_ctx):
+    """scripts/field_topology_dipole.py makes the calculator with __new__,
+    sets x, mu, the starts and the step limits, and calls
+    compute_topo_complete_c_shared_dipole: (L, 2) float32 rows in start
+    order, the same numbers as the batched call on the reference golden."""
+    import logging
+
+    from pycpet_amd import ops
+    from pycpet_amd.calculator import calculator
+
+    g = np.load(GOLDEN / "ref_topo_dipole.npz")
+    c = calculator.__new__(calculator)
+    c.n_samples = len(g["starts"])
+    c.step_size = float(g["h"])
+    c.x = g["x"].astype(np.float32)
+    c.mu = g["mu"].astype(np.float32)
+    c.dimensions = g["dims"].astype(np.float32)
+    c.random_start_points = g["starts"]
+    c.random_max_samples = g["n_iter"]
+    c.concur_slip = 4
+    c.log = logging.getLogger("dipole-script")
+    hist = c.compute_topo_complete_c_shared_dipole()
+    assert hist.dtype == np.float32 and hist.shape == (len(g["starts"]), 2)
+    batch = ops.topo_dipole(g["starts"], g["n_iter"], float(g["h"]), g["dims"], g["x"], g["mu"],
+                            ctx=gpu_ctx).feats.astype(np.float32)
+    np.testing.assert_array_equal(hist, batch)
+    np.testing.assert_array_equal(c.hist, hist)
+
+
+def test_field_topology_dipole_command(gpu_ctx, tmp_path, monkeypatch):
+    """The dipole-topology command on one .dip frame: the .top file holds
+    one row per start of the cube of starts, equal to the dipole kernel on
+    the same starts and step limits (regenerated from the same global numpy
+    seed), rotated into the box frame."""
+    from pycpet_amd import field_topology_dipole as FT
+    from pycpet_amd import ops
+    from pycpet_amd.field_ops import initialize_box_points_uniform
+
+    rng = np.random.default_rng(6)
+    D = 40
+    pos = rng.uniform(-6, 6, (D, 3))
+    pos = pos[np.linalg.norm(pos, axis=1) > 2.5]
+    mom = rng.standard_normal((len(pos), 3))
+    table = np.column_stack([np.arange(len(pos)), pos, mom])
+    (tmp_path / "in").mkdir()
+    np.savetxt(tmp_path / "in" / "frame_a.v1.dip", table, header="idx x y z mx my mz")
+    monkeypatch.chdir(tmp_path)
+    np.random.seed(7)
+    argv = ["-d", str(tmp_path / "in"), "-c", "0.5", "0", "0", "-x", "1.5", "0", "0",
+            "-y", "0.5", "1", "0", "-dim", "1.5", "1.5", "1.5", "-n", "60", "-s", "0.05"]
+    assert FT.main(argv) == 0
+    got = np.loadtxt(tmp_path / "frame_a.top")
+    assert got.shape == (64, 2)  # 60 starts round up to 4^3
+
+    np.random.seed(7)
+    dims = np.array([1.5, 1.5, 1.5], np.float32)
+    center = np.array([0.5, 0.0, 0.0])
+    starts, limits, T = initialize_box_points_uniform(
+        center=center, x=np.array([1.5, 0, 0.0]), y=np.array([0.5, 1, 0.0]), dimensions=dims,
+        N_cr=[4, 4, 4], dtype="float32", max_steps=round(2 * np.linalg.norm(dims) / 0.05),
+        ret_rand_max=True, inclusive=False, seed=None)
+    d = np.loadtxt(tmp_path / "in" / "frame_a.v1.dip", skiprows=1, dtype=np.float32)
+    Tinv = np.linalg.inv(T)
+    r = ops.topo_dipole(starts.reshape(-1, 3), limits, 0.05, dims,
+                        ((d[:, 1:4] - center) @ Tinv).astype(np.float32),
+                        (d[:, 4:7] @ Tinv).astype(np.float32), ctx=gpu_ctx)
+    np.testing.assert_array_equal(got, r.feats.astype(np.float32).astype(np.float64))

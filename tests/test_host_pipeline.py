@@ -371,3 +371,16 @@ def test_large_files_parse_in_pieces(tmp_path):
             np.empty(16 * n_rec, np.uint8)]  # kept alive across the call
     rc = lib.cpet_pqr_parse(str(q).encode(), n_rec, 16, *[b.ctypes.data for b in bufs])
     assert rc != 0 and f"line {at + 1}:" in nat.last_error(lib)
+
+
+def test_field_topology_dipole_command_checks(tmp_path, capsys):
+    """The dipole command's host checks: no .dip files -> nothing to do and
+    nothing printed (the reference's loop never runs); a table without 7
+    columns -> the reference's ValueError, before any GPU work."""
+    from pycpet_amd import field_topology_dipole as FT
+
+    assert FT.main(["-d", str(tmp_path)]) == 0
+    assert capsys.readouterr().out == ""
+    np.savetxt(tmp_path / "bad.dip", np.ones((4, 6)), header="h")
+    with pytest.raises(ValueError, match="7 columns"):
+        FT.main(["-d", str(tmp_path)])
