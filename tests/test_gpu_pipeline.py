@@ -224,3 +224,61 @@ def test_topo_driver_rk4_option(tmp_path, gpu_ctx):
         write_rows(str(want), rk, "%.18e")
         assert (tmp_path / "out" / f"frame{i}.top").read_bytes() == want.read_bytes()
         assert not np.array_equal(rk, eu)
+
+
+def test_residue_breakdown_command(tmp_path, gpu_ctx, monkeypatch):
+    """The residue breakdown over two frames: every residue's batched
+    topology histogram against the frame's full one, equal to running each
+    residue's charges alone through the calculator and numpy.histogram2d /
+    distance_numpy one residue at a time."""
+    import ast
+    import json
+
+    from pycpet_amd import residue_breakdown as RB
+    from pycpet_amd.calculator import calculator
+    from pycpet_amd.clusterhist import distance_numpy
+    from pycpet_amd.io import write_rows
+
+    (tmp_path / "in").mkdir()
+    (tmp_path / "tops").mkdir()
+    for k in range(2):
+        synthetic_frame(tmp_path / "in" / f"frame{k}.pdb", n=60, seed=30 + k)
+    opts = _options(tmp_path, "topo", n_samples=27, step_size=0.05,
+                    max_streamline_init="fixed_rand", dimensions=[0.5, 0.5, 0.5])
+    del opts["filter_radius"]  # leaves the residue lists aligned with the charges
+    (tmp_path / "opts.json").write_text(json.dumps(opts))
+    pdbs = sorted(p.name for p in (tmp_path / "in").iterdir())
+    for name in pdbs:
+        c = calculator(opts, str(tmp_path / "in" / name))
+        write_rows(tmp_path / "tops" / (name[:-4] + ".top"), c.compute_topo_complete_c_shared(),
+                   "%.18e")
+    monkeypatch.chdir(tmp_path)
+    bounds = ["-d", "0,1.2", "-dx", "0.1", "-c", "0,3", "-cx", "0.25"]
+    assert RB.main(["-t", str(tmp_path / "tops"), "-p", str(tmp_path / "in"),
+                    "-o", str(tmp_path / "opts.json")] + bounds) == 0
+
+    nd, nc = int((1.2 - 0.0) / float("0.1")), int((3.0 - 0.0) / float("0.25"))  # 11, 12
+    first = calculator(opts, str(tmp_path / "in" / pdbs[0]))
+    residues = RB.residue_keys(first.resids, first.residue_number)
+    assert len(residues) == 12  # 60 charges in residues of 5; the ligand is filtered out
+    per = {i: [] for i in range(len(residues))}
+    for name in pdbs:
+        c = calculator(opts, str(tmp_path / "in" / name))
+        orig = RB.get_hist_grid(np.loadtxt(tmp_path / "tops" / (name[:-4] + ".top")),
+                                [0, 1.2], nd, [0, 3], nc)
+        x, q = c.x, c.Q
+        for i, r in enumerate(residues):
+            idx = [j for j, n in enumerate(c.resids)
+                   if n == r["resname"] and c.residue_number[j] == r["resid"]]
+            c.x, c.Q = x[idx], q[idx]
+            grid = RB.get_hist_grid(c.compute_topo_complete_c_shared(), [0, 1.2], nd, [0, 3], nc)
+            per[i].append(distance_numpy(grid, orig))
+    want = sorted([[r, np.mean(per[i]), np.std(per[i])] for i, r in enumerate(residues)],
+                  key=lambda item: item[1])
+    got = [ast.literal_eval(line) for line in (tmp_path / "chi_squared_avg.txt").read_text()
+           .splitlines()]
+    assert [g[0] for g in got] == [RB._plain(w[0]) for w in want]
+    np.testing.assert_array_equal([g[1:] for g in got], [[float(w[1]), float(w[2])] for w in want])
+    listed = [ast.literal_eval(line) for line in (tmp_path / "residue_list.txt").read_text()
+              .splitlines()]
+    assert listed == [RB._plain(r) for r in residues]

@@ -384,3 +384,30 @@ def test_field_topology_dipole_command_checks(tmp_path, capsys):
     np.savetxt(tmp_path / "bad.dip", np.ones((4, 6)), header="h")
     with pytest.raises(ValueError, match="7 columns"):
         FT.main(["-d", str(tmp_path)])
+
+
+def test_residue_breakdown_host_rules(tmp_path):
+    """Residues are distinct (name, number) pairs in first-appearance order
+    without water; get_hist_grid is histogram2d over the bounds / total;
+    malformed bounds raise the reference's messages."""
+    from pycpet_amd import residue_breakdown as RB
+
+    names = ["ALA", "ALA", "WAT", "GLY", "ALA", "WAT", "GLY", "LYS"]
+    nums = [1, 1, 2, 3, 1, 2, 4, 5]
+    assert RB.residue_keys(names, nums) == [
+        {"resname": "ALA", "resid": 1}, {"resname": "GLY", "resid": 3},
+        {"resname": "GLY", "resid": 4}, {"resname": "LYS", "resid": 5}]
+    rows = np.random.default_rng(0).random((500, 2)) * [1.2, 2.5]
+    g = RB.get_hist_grid(rows, [0.0, 1.0], 10, [0.0, 2.0], 8)
+    a, _, _ = np.histogram2d(rows[:, 0], rows[:, 1], bins=[10, 8], range=[[0, 1], [0, 2]])
+    np.testing.assert_array_equal(g, (a / a.sum()).ravel())
+    opts = tmp_path / "o.json"
+    opts.write_text("{}")
+    base = ["-t", str(tmp_path), "-p", str(tmp_path), "-dx", "0.1", "-cx", "0.1", "-o", str(opts)]
+    with pytest.raises(ValueError, match="lists of 2 numbers"):
+        RB.main(base + ["-d", "0,1,2", "-c", "0,1"])
+    with pytest.raises(ValueError, match="Upper bound"):
+        RB.main(base + ["-d", "1,0", "-c", "0,1"])
+    with pytest.raises(ValueError, match="Options file not found"):
+        RB.main(["-t", ".", "-p", ".", "-d", "0,1", "-dx", "0.1", "-c", "0,1", "-cx", "0.1",
+                 "-o", str(tmp_path / "missing.json")])
