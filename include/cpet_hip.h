@@ -358,7 +358,11 @@ int cpet_ctx_refined_points(cpet_ctx *ctx, long long *n);
  *   streamline pair term (0 or 1: the direct form; 2: the expanded
  *   far-field form for frames whose charges are all far from the walk
  *   region, the direct form for the others; plain charges, 4 or more waves
- *   per workgroup). */
+ *   per workgroup).  Streamline results are the same bit for bit with 16
+ *   waves and with 1 wave per workgroup (the one wave walks the 16 waves'
+ *   charge slices in order); 2, 4 and 8 waves split the charges
+ *   differently and agree to FP32 rounding.  The batch default is 1 wave
+ *   when every frame has at most 128 charges, else 16. */
 int cpet_set_tuning(cpet_ctx *ctx, int kind, int a, int b);
 
 /* ------------------------------------------------------ cluster step

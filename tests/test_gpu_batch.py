@@ -136,3 +136,45 @@ def test_batch_device_tensors_match_host(gpu_ctx):
     for r, rt in zip(ra, rb):
         np.testing.assert_array_equal(r.feats, rt.feats.cpu().numpy())
         np.testing.assert_array_equal(r.steps, rt.steps.cpu().numpy())
+
+
+def test_one_wave_walk_equals_sixteen_waves(gpu_ctx):
+    """One-wave workgroups walk the 16 waves' charge slices in order, so
+    streamlines come out the same bit for bit as with 16 waves: single
+    frames of 1 to 1,000 charges (group- and tile-ragged counts), charges
+    and dipoles, FP64 sums and RK4; and the batch default for small frames
+    (one wave) against the single-frame default (16 waves)."""
+    from pycpet_amd import batch, ops
+    from pycpet_amd import workloads as W
+
+    d = np.array([0.6, 0.5, 0.55], np.float32)
+    st, it, _ = W.topo_starts(700, d, 0.01, 300, seed=3, start_seed=4)
+    try:
+        for N in (1, 3, 16, 61, 64, 65, 128, 131, 300, 1000):
+            x, q = W.ball_exclusion(N, 12.0, 0.6, 1.5, seed=200 + N)
+            if N == 1:
+                q = np.array([0.5], np.float32)  # the mean-free charge of one would be 0
+            mu = np.random.default_rng(N).standard_normal((N, 3)).astype(np.float32)
+            print(f"one-wave walk N={N}", flush=True)
+            runs = {}
+            for S in (16, 1):
+                gpu_ctx.set_tuning(2, S, 0)
+                runs[S] = [ops.topo(st, it, 0.01, d, x, q, ctx=gpu_ctx),
+                           ops.topo(st, it, 0.01, d, x, q, prec="f64", ctx=gpu_ctx),
+                           ops.topo(st, it, 0.01, d, x, q, integrator="rk4", ctx=gpu_ctx),
+                           ops.topo_dipole(st, it, 0.01, d, x, mu, ctx=gpu_ctx)]
+            for a, b in zip(runs[16], runs[1]):
+                np.testing.assert_array_equal(a.feats, b.feats)
+                np.testing.assert_array_equal(a.steps, b.steps)
+                np.testing.assert_array_equal(a.exited, b.exited)
+    finally:
+        gpu_ctx.set_tuning(2, 0, 0)
+    frames = []
+    for i, N in enumerate((5, 12, 40, 77, 128, 9)):
+        x, q = W.ball_exclusion(N, 12.0, 0.6, 1.5, seed=300 + i)
+        frames.append({"x": x, "q": q, "starts": st, "n_iter": it, "h": 0.01, "dims": d})
+    outs = batch.batch_topo(frames, ctx=gpu_ctx)  # every frame <= 128 charges: one wave
+    for f, r in zip(frames, outs):
+        r1 = ops.topo(f["starts"], f["n_iter"], f["h"], f["dims"], f["x"], f["q"], ctx=gpu_ctx)
+        np.testing.assert_array_equal(r.feats, r1.feats)
+        np.testing.assert_array_equal(r.steps, r1.steps)
