@@ -171,6 +171,10 @@ def aux_rows(ctx, T, case, cpu: bool, cores: int):
       dipole_topo_5k  point-dipole streamlines (thread_operation_dipole,
                       CPET/utils/math_module.c:593-659): the 5,000 positions of
                       the main workload as dipoles, the same 100,000 starts;
+      residue_breakdown_200x20k
+                      200 residues of 8-24 charges, each alone over the same
+                      20,000 starts, batched with 20x20 histograms
+                      (scripts/residue_breakdown_analysis.py:160-188);
       frame_io        PDB parse (CPET/utils/io.py:574-636) of a 200,000-atom
                       frame, .top writing (np.savetxt, CPET/source/CPET.py:123)
                       and reading of 1,000,000 rows;
@@ -181,7 +185,7 @@ def aux_rows(ctx, T, case, cpu: bool, cores: int):
 
     import torch
 
-    from pycpet_amd import clusterhist as CH, io as cio, ops
+    from pycpet_amd import clusterhist as CH, io as cio, ops, workloads as W
 
     out = {}
     # -- dipole streamlines
@@ -224,6 +228,56 @@ def aux_rows(ctx, T, case, cpu: bool, cores: int):
             row["cpu_reference"] = {"pair_evals_per_s": cp / dt, "cores": cores,
                                     "sample": f"{len(done)} streamlines, {dt:.1f} s"}
     out["dipole_topo_5k"] = row
+
+    # -- residue breakdown: each residue's few charges alone drive the same
+    # starts (scripts/residue_breakdown_analysis.py:160-188), all residues of
+    # a frame in batched launches with their 20x20 histograms
+    from pycpet_amd import batch as B
+
+    R, Lr = 200, 20_000
+    rst, rit, _ = W.topo_starts(Lr, case.dims, case.h, 2000, seed=61, start_seed=62)
+    rg = np.random.default_rng(31)
+    frames = []
+    for _ in range(R):
+        n = int(rg.integers(8, 25))
+        c = rg.standard_normal(3)
+        c *= rg.uniform(5.0, 14.0) / np.linalg.norm(c)
+        frames.append({"x": (c + rg.normal(0.0, 1.2, (n, 3))).astype(np.float32),
+                       "q": rg.normal(0.0, 0.4, n).astype(np.float32), "starts": rst,
+                       "n_iter": rit, "h": case.h, "dims": case.dims})
+    edges = (np.linspace(0.0, 3.2, 21), np.linspace(0.0, 40.0, 21))
+    B.batch_topo(frames[:4], edges=edges, ctx=ctx)
+    t_b, res = _best_of(lambda: B.batch_topo(frames, edges=edges, ctx=ctx), k=2)
+    rsteps = [r.steps.astype(np.int64) for r in res]
+    rpairs = float(sum((s_ + 4).sum() * len(f["q"]) for s_, f in zip(rsteps, frames)))
+    row = {"residues": R, "starts_per_residue": Lr, "charges_per_residue": "8-24",
+           "wall_s": t_b, "pair_evals_per_s": rpairs / t_b,
+           "streamlines_per_s": R * Lr / t_b,
+           "note": "host arrays in and out; the histograms counted in the launch"}
+    if cpu:
+        from concurrent.futures import ThreadPoolExecutor
+
+        from oracle import oracle as O
+
+        if O.Ref.available():
+            ref = O.Ref()
+            pick = np.random.default_rng(125).integers(0, [R, Lr], (200_000, 2))
+            done, pos = [], 0
+            t1 = time.perf_counter()
+            with ThreadPoolExecutor(max_workers=cores) as ex:
+                while time.perf_counter() - t1 < 4.0 and pos < len(pick):
+                    chunk = pick[pos:pos + 16 * cores]
+                    list(ex.map(lambda fi: ref.thread_operation(
+                        rst[fi[1]], rit[fi[1]], case.h, case.dims, frames[fi[0]]["x"],
+                        frames[fi[0]]["q"]), chunk))
+                    done.extend(chunk.tolist())
+                    pos += len(chunk)
+            dt = time.perf_counter() - t1
+            cp = float(sum((rsteps[f][i] + 4) * len(frames[f]["q"]) for f, i in done))
+            row["cpu_reference"] = {"pair_evals_per_s": cp / dt,
+                                    "streamlines_per_s": len(done) / dt, "cores": cores,
+                                    "sample": f"{len(done)} (residue, start) pairs, {dt:.1f} s"}
+    out["residue_breakdown_200x20k"] = row
 
     with tempfile.TemporaryDirectory() as tmp:
         # -- frame I/O
