@@ -361,8 +361,9 @@ int cpet_ctx_refined_points(cpet_ctx *ctx, long long *n);
  *   per workgroup).  Streamline results are the same bit for bit with 16
  *   waves and with 1 wave per workgroup (the one wave walks the 16 waves'
  *   charge slices in order); 2, 4 and 8 waves split the charges
- *   differently and agree to FP32 rounding.  The batch default is 1 wave
- *   when every frame has at most 128 charges, else 16. */
+ *   differently and agree to FP32 rounding.  The default is 1 wave for a
+ *   batch whose frames all have at most 128 charges and for a single frame
+ *   of at most 16, else 16. */
 int cpet_set_tuning(cpet_ctx *ctx, int kind, int a, int b);
 
 /* ------------------------------------------------------ cluster step

@@ -337,19 +337,24 @@ struct ChargeSrc {
   static constexpr int kComp = 3;  // the slice sums are the field components
   // Unsoftened field of charge records [r0, r1) at one point (the k factor
   // is applied by the caller).
+  // The FP32 terms of one group of four charge records, added into f.
+  __device__ static void group(float px, float py, float pz, const float4 (&c)[4],
+                               float (&f)[3]) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float dx = px - c[u].x, dy = py - c[u].y, dz = pz - c[u].z;
+      const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+      const float ri = __builtin_amdgcn_rsqf(r2);
+      const float s = (c[u].w * ri) * (ri * ri);
+      f[0] = fmaf(s, dx, f[0]);
+      f[1] = fmaf(s, dy, f[1]);
+      f[2] = fmaf(s, dz, f[2]);
+    }
+  }
   __device__ static void slice(float px, float py, float pz, const float4* __restrict__ ch,
                                int r0, int r1, double (&e)[3]) {
     stream_recs<kTile, 3>(ch, r0, r1, [&](const float4 (&c)[4], float (&f)[3]) {
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const float dx = px - c[u].x, dy = py - c[u].y, dz = pz - c[u].z;
-        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-        const float ri = __builtin_amdgcn_rsqf(r2);
-        const float s = (c[u].w * ri) * (ri * ri);
-        f[0] = fmaf(s, dx, f[0]);
-        f[1] = fmaf(s, dy, f[1]);
-        f[2] = fmaf(s, dz, f[2]);
-      }
+      group(px, py, pz, c, f);
     }, e);
   }
   // Sources [g0, g1), both multiples of 16.
@@ -391,22 +396,27 @@ struct DipoleSrc {
   __device__ static float coef(float mx, float my, float mz, float dx, float dy, float dz) {
     return PHYS ? 3.f * fmaf(mz, dz, fmaf(my, dy, mx * dx)) : fmaf(3.f * mx, dx, fmaf(my, dy, mz * dz));
   }
+  // The FP32 terms of one group of four records (two dipoles), added into f.
+  __device__ static void group(float px, float py, float pz, const float4 (&c)[4],
+                               float (&f)[3]) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const float4 X = c[2 * u], M = c[2 * u + 1];
+      const float dx = px - X.x, dy = py - X.y, dz = pz - X.z;
+      const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+      const float ri = __builtin_amdgcn_rsqf(r2);
+      const float ri2 = ri * ri;
+      const float ri5 = (ri2 * ri2) * ri;
+      const float cc = coef(M.x, M.y, M.z, dx, dy, dz);
+      f[0] = fmaf(ri5, fmaf(cc, dx, -M.x * r2), f[0]);
+      f[1] = fmaf(ri5, fmaf(cc, dy, -M.y * r2), f[1]);
+      f[2] = fmaf(ri5, fmaf(cc, dz, -M.z * r2), f[2]);
+    }
+  }
   __device__ static void slice(float px, float py, float pz, const float4* __restrict__ ch,
                                int r0, int r1, double (&e)[3]) {
     stream_recs<2 * kTile, 3>(ch, r0, r1, [&](const float4 (&c)[4], float (&f)[3]) {
-#pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        const float4 X = c[2 * u], M = c[2 * u + 1];
-        const float dx = px - X.x, dy = py - X.y, dz = pz - X.z;
-        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-        const float ri = __builtin_amdgcn_rsqf(r2);
-        const float ri2 = ri * ri;
-        const float ri5 = (ri2 * ri2) * ri;
-        const float cc = coef(M.x, M.y, M.z, dx, dy, dz);
-        f[0] = fmaf(ri5, fmaf(cc, dx, -M.x * r2), f[0]);
-        f[1] = fmaf(ri5, fmaf(cc, dy, -M.y * r2), f[1]);
-        f[2] = fmaf(ri5, fmaf(cc, dz, -M.z * r2), f[2]);
-      }
+      group(px, py, pz, c, f);
     }, e);
   }
   // Sources [g0, g1), both multiples of 16.
@@ -484,6 +494,23 @@ struct ChargeSrcF64 : ChargeSrc {
     e[2] = ez;
   }
 };
+// Policies whose FP32 slice is stream_recs over 4-record groups with a
+// group() body (the one-wave walk streams such groups directly).
+template <class Src>
+struct HasGroup {
+  static constexpr bool value = false;
+};
+template <>
+struct HasGroup<ChargeSrc> {
+  static constexpr bool value = true;
+};
+template <bool P>
+struct HasGroup<DipoleSrc<P>> {
+  static constexpr bool value = true;
+};
+template <class Base>
+struct HasGroup<Rk4<Base>> : HasGroup<Base> {};
+
 template <class Src>
 struct IsF64 {
   static constexpr bool value = false;
@@ -629,6 +656,7 @@ struct FrameView {
 // without changing any result.
 constexpr int kVirtWaves = 16;
 constexpr int kVirtMaxCharges = 128;  // batch default: one-wave walk up to this many
+constexpr int kVirtMaxSingle = 16;    // single-frame default: the same
 
 // Runs the workgroup's 64 slots until the frame's queue is exhausted.  Slot
 // state is identical in every wave: each wave sums its charge slice for all
@@ -642,6 +670,76 @@ constexpr int kVirtMaxCharges = 128;  // batch default: one-wave walk up to this
 // the queue for the next streamline (one idle iteration).  first_q >= 0
 // gives the slot an initial queue position, first_q < 0 starts it empty.
 // part holds kPartBufs(S) * S * 4 * 64 doubles, esum 4 * 64.
+// The one-wave walk, general form: the kVirtWaves slices in order, each
+// summed by the policy's slice exactly as its wave of the kVirtWaves-wave
+// split would, added into fc (which starts at 0) in that order.  An empty
+// slice would add +0.0 there, so it is skipped.
+template <class Src, int NC>
+__device__ __forceinline__ void walk_slices(float px, float py, float pz, double dpx, double dpy,
+                                            double dpz, const float4* __restrict__ ch,
+                                            const FrameView& f, double (&fc)[NC]) {
+  for (int v = 0; v < kVirtWaves; ++v) {
+    int r0, r1;
+    wave_records(f.vrec0, f.vnrec, v, kVirtWaves, r0, r1, f.vgran);
+    if (r0 >= r1) continue;
+    double pv[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) pv[c] = 0.0;
+    if constexpr (IsF64<Src>::value)
+      Src::slice64(dpx, dpy, dpz, ch, r0, r1, pv);
+    else
+      Src::slice(px, py, pz, ch, r0, r1, pv);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) fc[c] += pv[c];
+  }
+}
+
+// The one-wave walk when no slice holds more than one 4-record group (at
+// most kVirtWaves groups): the non-empty slices are the groups themselves,
+// in order, so the walk streams the groups with the next group's scalar
+// loads in flight during the current group's arithmetic (as stream_recs
+// does within a slice).  A slice's sum is 0.0 + x, x the FP64 value of its
+// group's FP32 sum, which differs from x only for x = -0; fc starts at +0
+// and holds sums of multiples of 2^-149, so it is never -0, and adding -0 or
+// +0 leaves it the same: adding x directly gives the 16-wave reduction's fc.
+template <class Src>
+__device__ __forceinline__ void walk_groups(float px, float py, float pz,
+                                            const float4* __restrict__ ch, int rec0, int nrec,
+                                            double (&fc)[3]) {
+  constexpr int kWaitLgkm0 = 0xc07f;  // s_waitcnt lgkmcnt(0)
+  const int ng = nrec / 4;
+  if (ng <= 0) return;
+  const int last = rec0 + 4 * (ng - 1);
+  int nx = rec0;
+  float4 a[4], b[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) a[u] = ch[nx + u];
+  auto one = [&](const float4 (&c)[4]) __attribute__((always_inline)) {
+    float g[3] = {0.f, 0.f, 0.f};
+    Src::group(px, py, pz, c, g);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) fc[k] += (double)g[k];
+  };
+  for (int i = 0; i < ng; i += 2) {
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+    nx = min(nx + 4, last);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) b[u] = ch[nx + u];
+    __builtin_amdgcn_sched_barrier(0);
+    one(a);
+    if (i + 1 >= ng) break;
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+    nx = min(nx + 4, last);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a[u] = ch[nx + u];
+    __builtin_amdgcn_sched_barrier(0);
+    one(b);
+  }
+  __builtin_amdgcn_s_waitcnt(kWaitLgkm0);  // no load left in flight
+}
+
 __host__ __device__ constexpr int kPartBufs(int S) { return S >= 4 ? 1 : 2; }
 
 template <int S, class Src>
@@ -696,22 +794,16 @@ __device__ __forceinline__ void run_slots(const float4* __restrict__ ch, const F
     const float ex32 = (float)(rk4 ? qx : px), ey32 = (float)(rk4 ? qy : py),
                 ez32 = (float)(rk4 ? qz : pz);
     if (st == kActive) {
-      if constexpr (S == 1) {
-        for (int v = 0; v < kVirtWaves; ++v) {
-          int r0, r1;
-          wave_records(f.vrec0, f.vnrec, v, kVirtWaves, r0, r1, f.vgran);
-          // an empty slice adds +0.0 in the 16-wave reduction: no change
-          if (r0 >= r1) continue;
-          double pv[NC];
-#pragma unroll
-          for (int c = 0; c < NC; ++c) pv[c] = 0.0;
-          if constexpr (IsF64<Src>::value)
-            Src::slice64(rk4 ? qx : px, rk4 ? qy : py, rk4 ? qz : pz, ch, r0, r1, pv);
-          else
-            Src::slice(ex32, ey32, ez32, ch, r0, r1, pv);
-#pragma unroll
-          for (int c = 0; c < NC; ++c) fc[c] += pv[c];
+      if constexpr (S == 1 && HasGroup<Src>::value) {
+        if (f.vnrec <= kVirtWaves * 4) {
+          walk_groups<Src>(ex32, ey32, ez32, ch, f.vrec0, f.vnrec, fc);
+        } else {
+          walk_slices<Src, NC>(ex32, ey32, ez32, rk4 ? qx : px, rk4 ? qy : py, rk4 ? qz : pz,
+                               ch, f, fc);
         }
+      } else if constexpr (S == 1) {
+        walk_slices<Src, NC>(ex32, ey32, ez32, rk4 ? qx : px, rk4 ? qy : py, rk4 ? qz : pz, ch,
+                             f, fc);
       } else {
         if constexpr (IsF64<Src>::value)
           Src::slice64(rk4 ? qx : px, rk4 ? qy : py, rk4 ? qz : pz, ch, f.r0, f.r1, fc);
@@ -1250,7 +1342,6 @@ int topo_run(cpet_ctx* ctx, int N, int L, const float* starts,
              int npad, double* feats, int32_t* steps, int32_t* exited, int nb_d,
              const double* edges_d, int nb_c, const double* edges_c,
              uint32_t* hist, uint32_t flags, int src) {
-  (void)N;
   const bool dev = (flags & CPET_F_DEVICE) != 0;
   hipStream_t st = ctx->stream;
   int rc;
@@ -1301,8 +1392,11 @@ int topo_run(cpet_ctx* ctx, int N, int L, const float* starts,
   }
 
   // far-field records and their safety flag (plain charges, S >= 4)
+  // 16 waves x 64 slots: fastest measured (S 4/8/16, G 256..2048); frames
+  // of at most kVirtMaxSingle charges: one-wave workgroups, the same results
+  // (kVirtWaves), 2.2x faster at 16 charges and 100,000 starts, slower at 64
   int S = ctx->tune[2][0];
-  if (S <= 0) S = 16;  // 16 waves x 64 slots: fastest measured (S 4/8/16, G 256..2048)
+  if (S <= 0) S = N <= kVirtMaxSingle ? 1 : 16;
   const float4* d_far = nullptr;
   const int* d_far_unsafe = nullptr;
   if (src == kSrcCharge && S >= 4 && ctx->tune[5][0] == 2) {
@@ -1336,7 +1430,8 @@ int topo_run(cpet_ctx* ctx, int N, int L, const float* starts,
   // One 16-wave workgroup per CU: the fewest slots that keep every SIMD fed
   // (4 waves each, with the charge loads prefetched), so the tail of long
   // streamlines still running after the queue drains is shortest.
-  if (G <= 0) G = std::min(gmax, std::max(1, ctx->num_cus * (S >= 16 ? 1 : 2)));
+  if (G <= 0)
+    G = std::min(gmax, std::max(1, ctx->num_cus * (S >= 16 ? 1 : S == 1 ? 24 : 2)));
   G = std::min(G, gmax);
   hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(256), 0, st, d_counter, 1LL,
                      (unsigned int)(G * 64));
@@ -1470,8 +1565,8 @@ int topo_batch_run(cpet_ctx* ctx, int F, const int32_t* n_starts, const float* s
   // 16 waves per workgroup, as the single-frame kernel; when every frame has
   // at most kVirtMaxCharges charges (padded), one-wave workgroups walking the
   // 16 slices in order (the same results, see kVirtWaves), 24 per CU (their
-  // LDS allows about that many): 2.9x faster on 16-charge frames, 1.1x at
-  // 128, slower from 256 on (200 to 6 frames of 20,000 starts).
+  // LDS allows about that many): 4.9x faster on 16-charge frames, 1.7x at
+  // 64, 1.1x at 128, slower from 256 on (200 to 6 frames of 20,000 starts).
   int maxpad = 0;
   for (int f = 0; f < F; ++f) maxpad = std::max(maxpad, chg_off[f + 1] - chg_off[f]);
   const int S = ctx->tune[3][0] > 0 ? ctx->tune[3][0] : (maxpad <= kVirtMaxCharges ? 1 : 16);
