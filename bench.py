@@ -31,8 +31,10 @@ charges) and `volume_esp_30k` (the same grid, potential only; both
 fraction and feature errors against it; --no-f64 drops the line) and
 `topo_rk4` (the same streamlines integrated with RK4, four field
 evaluations per step; --no-rk4).  `aux` (--no-aux): point-dipole
-streamlines, frame I/O and the cluster step, each beside its CPU
-counterpart.
+streamlines, the residue breakdown, frame I/O and the cluster step, each
+beside its CPU counterpart.  `forms` (--no-forms): frames/s of 32 mixed MD
+frames batched against per-frame calls, and p50/p99 latency of 10 s of
+Poisson serving at 200 req/s against one request at a time.
 """
 from __future__ import annotations
 
@@ -353,6 +355,32 @@ def aux_rows(ctx, T, case, cpu: bool, cores: int):
     return out
 
 
+def forms_rows() -> dict:
+    """Short samples of the two north-star forms, on the driver's own run:
+    batching (32 mixed MD frames, 16 volume and 16 streamline frames of
+    3k-40k charges: one volume and one persistent streamline launch on two
+    streams, default shapes, against the per-frame calls at their tuned
+    shapes) and serving (Poisson arrivals at 200 req/s for 10 s of the
+    120 s configuration, continuous batching against one request at a
+    time).  The full-size runs are `python -m pycpet_amd.framebench batch
+    --frames 512` and `serve --duration 120`."""
+    from pycpet_amd import framebench as FB
+
+    out = {}
+    b = FB.run_batch(32, 20_000, forms=("batched", "per_frame"), autotune=False)
+    out["batching_32_frames"] = {k: b[k] for k in (
+        "frames", "volume_frames", "topo_frames", "pair_evals", "batched_s",
+        "batched_frames_per_s", "per_frame_s", "per_frame_frames_per_s",
+        "batched_vs_per_frame", "batched_pair_evals_per_s")}
+    sv = FB.run_serve(200.0, 10.0, 4, plain_max_s=20.0)
+    keep = ("completed", "rejected", "frames_per_s", "p50_ms", "p99_ms", "wall_s")
+    out["serving_200rps_10s"] = {
+        "requests": sv["requests"], "mix": sv["mix"], "lanes": sv["lanes"],
+        "continuous_batching": {k: v for k, v in sv["continuous_batching"].items() if k in keep},
+        "one_at_a_time": {k: v for k, v in sv["plain_one_at_a_time"].items() if k in keep}}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -364,6 +392,7 @@ def main() -> int:
     ap.add_argument("--no-rk4", action="store_true")
     ap.add_argument("--no-f64", action="store_true")
     ap.add_argument("--no-aux", action="store_true")
+    ap.add_argument("--no-forms", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
@@ -552,6 +581,8 @@ def main() -> int:
     cores = min(16, avail)
     if not args.no_aux:
         out["aux"] = aux_rows(ctx, T, case, not args.no_cpu_baseline, cores)
+    if not args.no_forms:
+        out["forms"] = forms_rows()
 
     if not args.no_cpu_baseline and int(os.environ.get("RANK", "0")) == 0:
         out["cpu_baseline"] = cpu_baseline(case, steps_np, args.cpu_seconds, cores)

@@ -46,7 +46,14 @@ def _frames_to_device(frames, torch, dev):
     return out
 
 
-def run_batch(n_frames: int, topo_starts: int, seed: int = 3) -> dict:
+def run_batch(n_frames: int, topo_starts: int, seed: int = 3,
+              forms=("batched", "batched_graph", "per_frame", "plain"),
+              autotune: bool = True) -> dict:
+    """Frames/s of the mixed MD frame set in the chosen forms: batched (one
+    volume and one persistent topo launch on two streams), the same as one
+    HIP graph, the per-frame calls at their tuned shapes, and the de-tuned
+    plain per-frame shape; the batched launch shapes come from the
+    autotuner, or from the built-in defaults with autotune=False."""
     import torch
 
     from . import _native as nat, batch as B, ops, workloads as W
@@ -96,18 +103,23 @@ def run_batch(n_frames: int, topo_starts: int, seed: int = 3) -> dict:
     # launch shapes of the batched calls from the autotuner, timed on a subset
     from . import autotune as AT
 
-    kv = AT.tune_batch_volume(ctx, vol[:8])
-    kt = AT.tune_batch_topo(ctx2, top[:4])
-    ctx.set_profiling(False)
-    ctx2.set_profiling(False)
-    print(f"autotuned: batch volume K={kv[0]}, batch topo S,G={kt}", file=sys.stderr, flush=True)
+    kv, kt = (0, 0), (0, 0)
+    if autotune:
+        kv = AT.tune_batch_volume(ctx, vol[:8])
+        kt = AT.tune_batch_topo(ctx2, top[:4])
+        ctx.set_profiling(False)
+        ctx2.set_profiling(False)
+        print(f"autotuned: batch volume K={kv[0]}, batch topo S,G={kt}", file=sys.stderr,
+              flush=True)
 
-    # the same batched step captured into two HIP graphs (one per stream)
-    from . import graphs as G
+    gs = None
+    if "batched_graph" in forms:
+        # both batched launches as parallel branches of one HIP graph
+        from . import graphs as G
 
-    # both batched launches as parallel branches of one HIP graph
-    gs = G.CapturedStep([lambda c: B.batch_volume(vol, ctx=c), lambda c: B.batch_topo(top, ctx=c)],
-                        tuning=[{4: kv}, {3: kt}])
+        gs = G.CapturedStep([lambda c: B.batch_volume(vol, ctx=c),
+                             lambda c: B.batch_topo(top, ctx=c)],
+                            tuning=[{4: kv}, {3: kt}])
 
     def batched_graph():
         gs.replay()
@@ -117,6 +129,8 @@ def run_batch(n_frames: int, topo_starts: int, seed: int = 3) -> dict:
     for name, fn in (("batched", batched), ("batched_graph", batched_graph),
                      ("per_frame", lambda: per_frame(False)),
                      ("plain", lambda: per_frame(True))):
+        if name not in forms:
+            continue
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         fn()
@@ -127,29 +141,33 @@ def run_batch(n_frames: int, topo_starts: int, seed: int = 3) -> dict:
     tpairs = sum((int(r.steps.sum().item()) + 4 * r.steps.numel()) * f["q"].numel()
                  for r, f in zip(results["top"], top))
     out = {"workload": "md_frames", "frames": n_frames, "volume_frames": len(vol),
-           "autotuned": {"batch_volume_K": kv[0], "batch_topo_S_G": list(kt)},
+           "autotuned": ({"batch_volume_K": kv[0], "batch_topo_S_G": list(kt)}
+                         if autotune else None),
            "topo_frames": len(top), "topo_starts_per_frame": topo_starts,
            "pair_evals": vpairs + tpairs}
     for k, v in timings.items():
         out[f"{k}_s"] = v
         out[f"{k}_frames_per_s"] = n_frames / v
-    # the graph replay recomputes the eager batched results exactly; a
-    # second eager run separates run-to-run differences from graph ones
-    first = results.copy()
-    batched_graph()
-    batched()
-
     def maxdiff(a_list, b_list, get):
         return max(float((get(a) - get(b)).abs().max()) for a, b in zip(a_list, b_list))
 
-    out["eager_rerun_maxdiff"] = max(maxdiff(first["vol"], results["vol"], lambda r: r[0]),
-                                     maxdiff(first["top"], results["top"], lambda r: r.feats))
-    gv_out, gt_out = gs.outputs
-    out["graph_vs_eager_maxdiff"] = max(maxdiff(gv_out, results["vol"], lambda r: r[0]),
-                                        maxdiff(gt_out, results["top"], lambda r: r.feats))
-    out["batched_vs_plain"] = timings["plain"] / timings["batched"]
-    out["graph_vs_plain"] = timings["plain"] / timings["batched_graph"]
-    out["batched_vs_per_frame"] = timings["per_frame"] / timings["batched"]
+    if gs is not None:
+        # the graph replay recomputes the eager batched results exactly; a
+        # second eager run separates run-to-run differences from graph ones
+        first = results.copy()
+        batched_graph()
+        batched()
+        out["eager_rerun_maxdiff"] = max(maxdiff(first["vol"], results["vol"], lambda r: r[0]),
+                                         maxdiff(first["top"], results["top"], lambda r: r.feats))
+        gv_out, gt_out = gs.outputs
+        out["graph_vs_eager_maxdiff"] = max(maxdiff(gv_out, results["vol"], lambda r: r[0]),
+                                            maxdiff(gt_out, results["top"], lambda r: r.feats))
+    if "plain" in timings:
+        out["batched_vs_plain"] = timings["plain"] / timings["batched"]
+    if "plain" in timings and "batched_graph" in timings:
+        out["graph_vs_plain"] = timings["plain"] / timings["batched_graph"]
+    if "per_frame" in timings:
+        out["batched_vs_per_frame"] = timings["per_frame"] / timings["batched"]
     out["batched_pair_evals_per_s"] = (vpairs + tpairs) / timings["batched"]
     return out
 
