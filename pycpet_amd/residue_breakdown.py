@@ -81,9 +81,14 @@ def residue_histograms(calc, residues: Sequence[dict], edges, ctx=None) -> List[
     nums = list(calc.residue_number)
     starts = np.ascontiguousarray(calc.random_start_points, np.float32).reshape(-1, 3)
     n_iter = np.ascontiguousarray(calc.random_max_samples, np.int32).ravel()
+    # the atoms of each (name, number) pair, in order, in one pass (the
+    # reference scans all atoms per residue; dict lookup compares the same way)
+    members = {}
+    for i, key in enumerate(zip(names, nums)):
+        members.setdefault(key, []).append(i)
     jobs, out = [], [None] * len(residues)
     for k, res in enumerate(residues):
-        idx = [i for i, rn in enumerate(names) if rn == res["resname"] and nums[i] == res["resid"]]
+        idx = members.get((res["resname"], res["resid"]), [])
         if idx:
             jobs.append((k, {"x": calc.x[idx], "q": np.asarray(calc.Q)[idx], "starts": starts,
                              "n_iter": n_iter, "h": calc.step_size,
