@@ -234,7 +234,7 @@ def aux_rows(ctx, T, case, cpu: bool, cores: int):
     # -- residue breakdown: each residue's few charges alone drive the same
     # starts (scripts/residue_breakdown_analysis.py:160-188), all residues of
     # a frame in batched launches with their 20x20 histograms
-    from pycpet_amd import batch as B
+    from pycpet_amd import residue_breakdown as RB
 
     R, Lr = 200, 20_000
     rst, rit, _ = W.topo_starts(Lr, case.dims, case.h, 2000, seed=61, start_seed=62)
@@ -248,14 +248,17 @@ def aux_rows(ctx, T, case, cpu: bool, cores: int):
                        "q": rg.normal(0.0, 0.4, n).astype(np.float32), "starts": rst,
                        "n_iter": rit, "h": case.h, "dims": case.dims})
     edges = (np.linspace(0.0, 3.2, 21), np.linspace(0.0, 40.0, 21))
-    B.batch_topo(frames[:4], edges=edges, ctx=ctx)
-    t_b, res = _best_of(lambda: B.batch_topo(frames, edges=edges, ctx=ctx), k=2)
-    rsteps = [r.steps.astype(np.int64) for r in res]
-    rpairs = float(sum((s_ + 4).sum() * len(f["q"]) for s_, f in zip(rsteps, frames)))
+    xs, qs = [f["x"] for f in frames], [f["q"] for f in frames]
+    count = lambda xs_, qs_: RB.residue_counts(xs_, qs_, rst, rit, case.h, case.dims, edges,
+                                               ctx, return_steps=True)
+    count(xs[:4], qs[:4])
+    t_b, (_, rsum) = _best_of(lambda: count(xs, qs), k=2)
+    rpairs = float(sum((int(s_) + 4 * Lr) * len(q) for s_, q in zip(rsum, qs)))
     row = {"residues": R, "starts_per_residue": Lr, "charges_per_residue": "8-24",
            "wall_s": t_b, "pair_evals_per_s": rpairs / t_b,
            "streamlines_per_s": R * Lr / t_b,
-           "note": "host arrays in and out; the histograms counted in the launch"}
+           "note": "residue_breakdown.residue_counts: host charge sets and starts in, the "
+                   "histogram counts out; the starts uploaded once"}
     if cpu:
         from concurrent.futures import ThreadPoolExecutor
 
@@ -275,7 +278,13 @@ def aux_rows(ctx, T, case, cpu: bool, cores: int):
                     done.extend(chunk.tolist())
                     pos += len(chunk)
             dt = time.perf_counter() - t1
-            cp = float(sum((rsteps[f][i] + 4) * len(frames[f]["q"]) for f, i in done))
+            # per-streamline steps of the sampled residues, from one host-array batch
+            sampled = sorted({f for f, _ in done})
+            from pycpet_amd import batch as B
+
+            sres = B.batch_topo([frames[f] for f in sampled], ctx=ctx)
+            ssteps = {f: r.steps.astype(np.int64) for f, r in zip(sampled, sres)}
+            cp = float(sum((ssteps[f][i] + 4) * len(frames[f]["q"]) for f, i in done))
             row["cpu_reference"] = {"pair_evals_per_s": cp / dt,
                                     "streamlines_per_s": len(done) / dt, "cores": cores,
                                     "sample": f"{len(done)} (residue, start) pairs, {dt:.1f} s"}

@@ -18,10 +18,11 @@ residue_list.txt, in the working directory.
 
 On the GPU, a frame's residues run as batched streamline launches
 (cpet_batch_topo, one frame per residue, up to a bounded number of
-streamlines per launch), and each residue's histogram is counted in the
-launch's epilogue, exactly (the float32-rounded features, numpy's bin
-rule).  Where the reference constructs a ValueError for a missing options
-file or malformed bounds without raising it, this raises it.
+streamlines per launch) on device tensors, the shared starts uploaded once,
+and each residue's histogram is counted in the launch's epilogue, exactly
+(the float32-rounded features, numpy's bin rule).  Where the reference
+constructs a ValueError for a missing options file or malformed bounds
+without raising it, this raises it.
 """
 from __future__ import annotations
 
@@ -71,36 +72,72 @@ def residue_keys(resnames: Sequence, resnums: Sequence) -> List[dict]:
     return out
 
 
+def residue_counts(xs: Sequence[np.ndarray], qs: Sequence[np.ndarray], starts, n_iter, h,
+                   dims, edges, ctx=None, return_steps: bool = False):
+    """(R, nd, nc) exact histogram counts of the streamlines that each charge
+    set (xs[r], qs[r]) drives from the same starts and step limits: batched
+    launches (cpet_batch_topo) on device tensors, the starts uploaded once
+    and repeated on the GPU, each launch up to a bounded number of
+    streamlines, only the counts copied back.  With return_steps, also each
+    set's total Euler steps (R,) int64."""
+    import torch
+
+    from . import _native as nat, batch
+
+    ctx = ctx or nat.default_context()
+    dev = torch.device("cuda", ctx.device)
+    st = torch.from_numpy(np.ascontiguousarray(starts, np.float32).reshape(-1, 3)).to(dev)
+    it = torch.from_numpy(np.ascontiguousarray(n_iter, np.int32).ravel()).to(dev)
+    sizes = [len(q) for q in qs]
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    X = torch.from_numpy(np.ascontiguousarray(np.concatenate(xs) if xs else np.zeros((0, 3)),
+                                              np.float32).reshape(-1, 3)).to(dev)
+    Q = torch.from_numpy(np.ascontiguousarray(np.concatenate(qs) if qs else np.zeros(0),
+                                              np.float32).ravel()).to(dev)
+    dims = np.ascontiguousarray(dims, np.float32).ravel()
+    nd, nc = len(edges[0]) - 1, len(edges[1]) - 1
+    out = np.zeros((len(qs), nd, nc), np.float64)
+    steps = np.zeros(len(qs), np.int64)
+    per_launch = max(1, _MAX_STREAMLINES_PER_LAUNCH // max(1, st.shape[0]))
+    for c0 in range(0, len(qs), per_launch):
+        rs = range(c0, min(len(qs), c0 + per_launch))
+        frames = [{"x": X[offs[r]:offs[r + 1]], "q": Q[offs[r]:offs[r + 1]], "starts": st,
+                   "n_iter": it, "h": h, "dims": dims} for r in rs]
+        res = batch.batch_topo(frames, edges=edges, ctx=ctx)
+        out[c0:c0 + len(rs)] = torch.stack([r.hist for r in res]).cpu().numpy()
+        if return_steps:
+            steps[c0:c0 + len(rs)] = torch.stack(
+                [r.steps.to(torch.int64).sum() for r in res]).cpu().numpy()
+    return (out, steps) if return_steps else out
+
+
 def residue_histograms(calc, residues: Sequence[dict], edges, ctx=None) -> List[Optional[np.ndarray]]:
     """For each residue, the normalised (flattened) histogram of the
     topology driven by that residue's charges alone over calc's starts, or
     None for a residue with no charges in this frame."""
-    from . import batch
-
     names = list(calc.resids)
     nums = list(calc.residue_number)
-    starts = np.ascontiguousarray(calc.random_start_points, np.float32).reshape(-1, 3)
-    n_iter = np.ascontiguousarray(calc.random_max_samples, np.int32).ravel()
     # the atoms of each (name, number) pair, in order, in one pass (the
     # reference scans all atoms per residue; dict lookup compares the same way)
     members = {}
     for i, key in enumerate(zip(names, nums)):
         members.setdefault(key, []).append(i)
-    jobs, out = [], [None] * len(residues)
+    ks, xs, qs = [], [], []
+    Qall = np.asarray(calc.Q).reshape(-1)
     for k, res in enumerate(residues):
         idx = members.get((res["resname"], res["resid"]), [])
         if idx:
-            jobs.append((k, {"x": calc.x[idx], "q": np.asarray(calc.Q)[idx], "starts": starts,
-                             "n_iter": n_iter, "h": calc.step_size,
-                             "dims": calc.dimensions}))
-    per_launch = max(1, _MAX_STREAMLINES_PER_LAUNCH // max(1, len(starts)))
-    for c0 in range(0, len(jobs), per_launch):
-        chunk = jobs[c0:c0 + per_launch]
-        results = batch.batch_topo([f for _, f in chunk], edges=edges, ctx=ctx)
-        for (k, _), r in zip(chunk, results):
-            a = np.asarray(r.hist, np.float64)
-            with np.errstate(invalid="ignore"):
-                out[k] = (a / np.sum(a)).flatten()
+            ks.append(k)
+            xs.append(np.asarray(calc.x)[idx])
+            qs.append(Qall[idx])
+    out: List[Optional[np.ndarray]] = [None] * len(residues)
+    if not ks:
+        return out
+    counts = residue_counts(xs, qs, calc.random_start_points, calc.random_max_samples,
+                            calc.step_size, calc.dimensions, edges, ctx)
+    for k, a in zip(ks, counts):
+        with np.errstate(invalid="ignore"):
+            out[k] = (a / np.sum(a)).flatten()
     return out
 
 
