@@ -11,9 +11,11 @@ runs apply it without searching:
   batch volume   kind 4: points per lane (1,2,4)
 
 Buckets are powers of two of the charge count and of the point / streamline
-count, so frames of similar size share a choice.  Every candidate computes
-the same per-pair arithmetic; only the charge-split candidates regroup FP64
-partial sums.
+count, so frames of similar size share a choice.  Grid candidates compute the
+same per-pair arithmetic (charge splits regroup FP64 partial sums).
+Streamline candidates with 1 or 16 waves per workgroup give the same results
+bit for bit (the one wave walks the 16 waves' charge slices in order); 2, 4
+and 8 waves slice the charges differently, and agree to FP32 rounding.
 """
 from __future__ import annotations
 
@@ -87,7 +89,8 @@ def tune_topo(ctx: nat.Context, starts, n_iter, h, dims, x, q, *, cache: Optiona
         a, b = cache[k]
         ctx.set_tuning(2, a, b)
         return a, b
-    cands = [(s, g) for s in waves for g in groups]
+    # one-wave workgroups (same results as 16 waves) at their default count
+    cands = [(1, 0)] + [(s, g) for s in waves if s != 1 for g in groups]
     best, _ = sweep(ctx, 2, cands, lambda: ops.topo(starts, n_iter, h, dims, x, q, ctx=ctx),
                     phase=1)
     if cache is not None:
@@ -129,7 +132,8 @@ def tune_batch_topo(ctx: nat.Context, frames, *, cache: Optional[dict] = None,
         a, b = cache[k]
         ctx.set_tuning(3, a, b)
         return a, b
-    cands = [(w, g) for w in waves for g in groups]
+    # one-wave workgroups (same results as 16 waves) at their default count
+    cands = [(1, 0)] + [(w, g) for w in waves if w != 1 for g in groups]
     best, _ = sweep(ctx, 3, cands, lambda: B.batch_topo(frames, ctx=ctx), phase=0)
     if cache is not None:
         cache[k] = list(best)

@@ -178,3 +178,39 @@ def test_one_wave_walk_equals_sixteen_waves(gpu_ctx):
         r1 = ops.topo(f["starts"], f["n_iter"], f["h"], f["dims"], f["x"], f["q"], ctx=gpu_ctx)
         np.testing.assert_array_equal(r.feats, r1.feats)
         np.testing.assert_array_equal(r.steps, r1.steps)
+
+
+def test_autotuned_batch_shape(gpu_ctx):
+    """tune_batch_topo times the candidates on real frames and leaves the
+    fastest applied; the tuned launch gives the default launch's results
+    (bit for bit with 1 or 16 waves, to FP32 rounding otherwise), and a
+    cached choice is applied without timing again."""
+    from pycpet_amd import autotune as AT
+    from pycpet_amd import batch
+    from pycpet_amd import workloads as W
+
+    d = np.array([0.6, 0.5, 0.55], np.float32)
+    st, it, _ = W.topo_starts(3000, d, 0.01, 300, seed=5, start_seed=6)
+    frames = []
+    for i, N in enumerate((9, 20, 33, 48)):
+        x, q = W.ball_exclusion(N, 12.0, 0.6, 1.5, seed=400 + i)
+        frames.append({"x": x, "q": q, "starts": st, "n_iter": it, "h": 0.01, "dims": d})
+    base = batch.batch_topo(frames, ctx=gpu_ctx)
+    cache = {}
+    try:
+        S, G = AT.tune_batch_topo(gpu_ctx, frames, cache=cache)
+        print(f"autotuned S={S} G={G}", flush=True)
+        assert (S, G) == (1, 0) or S in (4, 8, 16)
+        tuned = batch.batch_topo(frames, ctx=gpu_ctx)
+        for a, b in zip(base, tuned):
+            if S in (1, 16):
+                np.testing.assert_array_equal(a.feats, b.feats)
+            else:
+                same = a.steps == b.steps
+                assert same.mean() >= 0.99
+                np.testing.assert_allclose(a.feats[same], b.feats[same], rtol=1e-4, atol=1e-9)
+        assert list(cache.values()) == [[S, G]]
+        gpu_ctx.set_tuning(3, 0, 0)
+        assert AT.tune_batch_topo(gpu_ctx, frames, cache=cache) == (S, G)
+    finally:
+        gpu_ctx.set_tuning(3, 0, 0)
