@@ -180,6 +180,9 @@ def aux_rows(ctx, T, case, cpu: bool, cores: int):
       frame_io        PDB parse (CPET/utils/io.py:574-636) of a 200,000-atom
                       frame, .top writing (np.savetxt, CPET/source/CPET.py:123)
                       and reading of 1,000,000 rows;
+      driver_32_frames
+                      the options-file driver (CPET/source/CPET.py:96-170)
+                      end to end over 32 frame files, topo and volume;
       cluster_step    make_histograms + construct_distance_matrix
                       (CPET/utils/calculator.py:596-718,1000-1015) over 256
                       frames of 20,000 features."""
@@ -322,6 +325,34 @@ def aux_rows(ctx, T, case, cpu: bool, cores: int):
             "top_read_equal": bool(np.array_equal(back, back_np)),
             "python_side": "pdb: the line-by-line restatement of the reference parser in "
                            "pycpet_amd.io; .top: numpy.savetxt / numpy.loadtxt"}
+
+        # -- the options-file frame driver (CPET.run): parse, filters and box
+        # frame, compute, and the output file of each of 32 frame files
+        from pycpet_amd.CPET import CPET
+
+        ind = os.path.join(tmp, "frames")
+        os.mkdir(ind)
+        nd_atoms = 10_000
+        for k in range(32):
+            xk, qk = W.ball_exclusion(nd_atoms, 35.0, 1.5, 3.0, seed=900 + k)
+            _write_pdb(os.path.join(ind, f"frame{k:02d}.pdb"), xk.astype(np.float64), qk)
+        base = {"inputpath": ind, "center": [0.0, 0.0, 0.0], "x": [1.0, 0.0, 0.0],
+                "y": [0.0, 1.0, 0.0]}
+        drv = {}
+        for method, extra in (("topo", {"dimensions": [1.5, 1.5, 1.5], "step_size": 0.005,
+                                        "n_samples": 10_000, "max_streamline_init": "fixed_rand"}),
+                              ("volume", {"dimensions": [1.0, 1.0, 1.0], "step_size": 0.1})):
+            opts = dict(base, CPET_method=method, outputpath=os.path.join(tmp, f"out_{method}"),
+                        **extra)
+            t1 = time.perf_counter()
+            CPET(opts).run()
+            dt = time.perf_counter() - t1
+            drv[method] = {"frames": 32, "atoms_per_frame": nd_atoms, "wall_s": dt,
+                           "frames_per_s": 32 / dt, **extra}
+        drv["note"] = ("CPET(options).run() end to end: 32 PDB files parsed, box frame, "
+                       "topo in batched launches of 16 frames (10,648 starts each) or a 21^3 "
+                       "field grid per frame, each frame's .top / _efield.dat written")
+        out["driver_32_frames"] = drv
 
     # -- cluster step: per-frame features from a mixture, like .top files
     F, Lf = 256, 20_000
