@@ -65,15 +65,16 @@ class calculator:
             (self.x, self.Q, self.atom_number, self.resids, self.residue_number,
              self.atom_type, self.chains) = parse_pdb(self.path_to_pdb, get_charges=True)
 
-        # ID rows as tuples of the arrays' elements (zip yields the same numpy
-        # scalars as indexing, 2.5x faster than an indexed comprehension)
+        # ID rows (atom number, name, residue name, residue number[, chain]) of
+        # the parsed arrays; built on first use (the filters), see ID below
         if hasattr(self, "chains"):
-            self.ID = list(zip(self.atom_number, self.atom_type, self.resids,
-                               self.residue_number, self.chains))
+            self._id_cols = (self.atom_number, self.atom_type, self.resids,
+                             self.residue_number, self.chains)
         else:
-            self.ID = list(zip(self.atom_number, self.atom_type, self.resids,
-                               self.residue_number))
+            self._id_cols = (self.atom_number, self.atom_type, self.resids,
+                             self.residue_number)
             self.chains = [None] * len(self.x)
+        self._ID = None
 
         # centre and axes (CPET/source/calculator.py:152-206)
         if type(options["center"]) == list:
@@ -200,6 +201,24 @@ class calculator:
             if self.dimensions is not None:
                 self.dimensions = self.dimensions.astype(np.float32)
         self.log.info("Initialized calculator: %d charges", len(self.Q))
+
+    @property
+    def ID(self):
+        """List of per-atom ID tuples, as the reference's constructor builds it
+        from the parsed columns (CPET/source/calculator.py:125-145); a
+        filter replaces it with its own result.  Built when first read (zip
+        yields the same numpy scalars as indexing the columns), so frames
+        that no filter touches skip it."""
+        if getattr(self, "_ID", None) is None:
+            cols = getattr(self, "_id_cols", None)
+            if cols is None:  # made with __new__ and never given an ID
+                raise AttributeError("'calculator' object has no attribute 'ID'")
+            self._ID = list(zip(*cols))
+        return self._ID
+
+    @ID.setter
+    def ID(self, value):
+        self._ID = value
 
     def _write_transformed_pdb(self, path_to_pdb, Tinv):
         if self.strip_filter:

@@ -411,3 +411,24 @@ def test_residue_breakdown_host_rules(tmp_path):
     with pytest.raises(ValueError, match="Options file not found"):
         RB.main(["-t", ".", "-p", ".", "-d", "0,1", "-dx", "0.1", "-c", "0,1", "-cx", "0.1",
                  "-o", str(tmp_path / "missing.json")])
+
+
+def test_calculator_id_rows(tmp_path):
+    """calculator.ID holds the reference's per-atom tuples of the parsed
+    columns (built when first read, from the columns as parsed), and a filter
+    replaces it with its own result."""
+    from pycpet_amd.calculator import calculator
+
+    synthetic_frame(tmp_path / "f.pdb", n=60, seed=2)
+    opts = {"CPET_method": "volume", "center": [10.0, -4.0, 7.5], "x": [11.0, -4.0, 7.5],
+            "y": [10.0, -3.0, 7.5], "dimensions": [1.0, 1.0, 1.0], "step_size": 0.5,
+            "frame_backend": "numpy"}
+    c = calculator(opts, str(tmp_path / "f.pdb"))
+    want = [(c.atom_number[i], c.atom_type[i], c.resids[i], c.residue_number[i], c.chains[i])
+            for i in range(len(c.x))]
+    c.atom_number = None  # a later reassignment does not change the parsed rows
+    got = c.ID
+    assert got == want and all(type(a) is type(b) for r, s in zip(got, want) for a, b in zip(r, s))
+    assert c.ID is got
+    f = calculator(dict(opts, filter={"resid": ["LIG"]}), str(tmp_path / "f.pdb"))
+    assert len(f.ID) == len(f.x) == 60 and all(row[2] != "LIG" for row in f.ID)
