@@ -369,6 +369,18 @@ def aux_rows(ctx, T, case, cpu: bool, cores: int):
     t_d, _ = _best_of(lambda: CH.construct_distance_matrix(hist, ctx=ctx))
     row = {"frames": F, "features_per_frame": Lf, "bins": [len(edges[0]) - 1, len(edges[1]) - 1],
            "histograms_s": t_h, "edges_alone_s": t_e, "chi2_matrix_s": t_d}
+    # the same step from .top files, as make_histograms takes it (rows written
+    # as the topo driver writes them)
+    topdir = tempfile.mkdtemp()
+    files = []
+    for k, f in enumerate(feats):
+        path = os.path.join(topdir, f"frame{k:03d}.top")
+        cio.write_rows(path, f, fmt="%.18e")
+        files.append(path)
+    CH.make_histograms(files[:2], ctx=ctx)
+    t_hf, hist_f = _best_of(lambda: CH.make_histograms(files, ctx=ctx), k=2)
+    row["histograms_from_files_s"] = t_hf
+    row["from_files_equal_in_memory"] = bool(np.array_equal(hist_f, hist))
     if cpu:
         import scipy.stats  # noqa: F401  imported at module load by the reference
 
@@ -391,6 +403,17 @@ def aux_rows(ctx, T, case, cpu: bool, cores: int):
                       f"frames in {t_od:.2f} s, scaled to {F} frames",
             "how": "the reference's steps with numpy.histogram2d / scipy iqr and sklearn "
                    "pairwise_distances over the chi2 function, one core"}
+        nf = 16
+        t1 = time.perf_counter()
+        O.make_histograms_files(files[:nf])
+        t_of = time.perf_counter() - t1
+        row["cpu_reference"]["histograms_from_files_s_scaled_to_all"] = t_of * F / nf
+        row["cpu_reference"]["from_files_sample"] = (
+            f"make_histograms' three passes over {nf} .top files in {t_of:.2f} s, scaled "
+            f"to {F}")
+    import shutil
+
+    shutil.rmtree(topdir, ignore_errors=True)
     out["cluster_step"] = row
     return out
 

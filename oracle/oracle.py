@@ -416,6 +416,51 @@ def cluster_histograms(features):
     return np.array(out), edges
 
 
+def make_histograms_files(topo_files):
+    """make_histograms (CPET/utils/calculator.py:596-718) from .top files,
+    with the reference's file passes: a line count per file (:603-606), a
+    parse of every row into the global arrays (:618-649), then per file a
+    second parse for its histogram (:690-707); bins from scipy iqr and the
+    global min / max.  Returns (F, B).  Timed as the CPU counterpart of the
+    cluster step from files; its arithmetic is cluster_histograms'."""
+    from scipy.stats import iqr
+
+    lens = np.zeros(len(topo_files), dtype=int)
+    for k, path in enumerate(topo_files):
+        with open(path) as fh:
+            lens[k] = sum(1 for line in fh if not line.startswith("#"))
+    d_all = np.zeros(int(lens.sum()))
+    c_all = np.zeros(int(lens.sum()))
+    pos = 0
+    for k, path in enumerate(topo_files):
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("#"):
+                    continue
+                parts = line.strip().split()
+                d_all[pos] = float(parts[0])
+                c_all[pos] = float(parts[1])
+                pos += 1
+    n = lens[0] if np.all(lens == lens[0]) else np.mean(lens)
+    dmax, cmax, dmin, cmin = np.max(d_all), np.max(c_all), np.min(d_all), np.min(c_all)
+    nd = int((dmax - dmin) / (2 * iqr(d_all) / (n ** (1 / 3))))
+    nc = int((cmax - cmin) / (2 * iqr(c_all) / (n ** (1 / 3))))
+    out = []
+    for path in topo_files:
+        dist, curv = [], []
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("#"):
+                    continue
+                parts = line.split()
+                dist.append(float(parts[0]))
+                curv.append(float(parts[1]))
+        a, _, _ = np.histogram2d(dist, curv, bins=[nd, nc],
+                                 range=[[dmin, dmax], [cmin, cmax]])
+        out.append((a / np.sum(a)).flatten())
+    return np.array(out)
+
+
 def cluster_histograms_mem(features):
     """make_histograms_mem (CPET/utils/calculator.py:721-845) on in-memory
     per-frame (L, 2) features: Python min / max over the rows, scipy iqr,

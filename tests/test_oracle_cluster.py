@@ -182,3 +182,16 @@ def test_mem_reader_commas_and_whitespace(tmp_path):
     q = tmp_path / "w.top"
     q.write_text("# header\n1.5 2.25\n3 4 99\n")
     np.testing.assert_array_equal(CH._read_topo_any_sep(q), [[1.5, 2.25], [3.0, 4.0]])
+
+
+def test_oracle_files_path_equals_in_memory(oracle_mod, tmp_path):
+    """The from-files restatement (three passes over each .top file) gives
+    the in-memory oracle's histograms bit for bit on savetxt-written rows."""
+    feats = _frames(4, F=3, L=400)
+    files = []
+    for k, f in enumerate(feats):
+        p = tmp_path / f"f{k}.top"
+        np.savetxt(p, f)
+        files.append(str(p))
+    want, _ = oracle_mod.cluster_histograms(feats)
+    np.testing.assert_array_equal(oracle_mod.make_histograms_files(files), want)
