@@ -341,7 +341,8 @@ def aux_rows(ctx, T, case, cpu: bool, cores: int):
         drv = {}
         for method, extra in (("topo", {"dimensions": [1.5, 1.5, 1.5], "step_size": 0.005,
                                         "n_samples": 10_000, "max_streamline_init": "fixed_rand"}),
-                              ("volume", {"dimensions": [1.0, 1.0, 1.0], "step_size": 0.1})):
+                              ("volume", {"dimensions": [1.0, 1.0, 1.0], "step_size": 0.1}),
+                              ("point_field", {"step_size": 0.1})):
             opts = dict(base, CPET_method=method, outputpath=os.path.join(tmp, f"out_{method}"),
                         **extra)
             t1 = time.perf_counter()
@@ -350,8 +351,9 @@ def aux_rows(ctx, T, case, cpu: bool, cores: int):
             drv[method] = {"frames": 32, "atoms_per_frame": nd_atoms, "wall_s": dt,
                            "frames_per_s": 32 / dt, **extra}
         drv["note"] = ("CPET(options).run() end to end: 32 PDB files parsed, box frame, "
-                       "topo in batched launches of 16 frames (10,648 starts each) or a 21^3 "
-                       "field grid per frame, each frame's .top / _efield.dat written")
+                       "topo in batched launches of 16 frames (10,648 starts each), a 21^3 "
+                       "field grid per frame, or the field at the box centre; each frame's "
+                       ".top / _efield.dat written, or one point_field.dat line per frame")
         out["driver_32_frames"] = drv
 
     # -- cluster step: per-frame features from a mixture, like .top files
