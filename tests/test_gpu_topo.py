@@ -157,6 +157,29 @@ def test_edge_cases(gpu_ctx, oracle_mod):
         ops.topo(starts, n_iter, 0.01, np.array([1.0, -1.0, 1.0], np.float32), x, q, ctx=gpu_ctx)
 
 
+def test_boundary_starts(gpu_ctx, oracle_mod):
+    """Starts outside the box (they leave after one step), exactly on faces,
+    edges and a corner (inside: the exit test is strict), and just inside,
+    against the FP64 oracle (itself pinned on these starts to the
+    reference C): step counts and stop kinds exact, features to 1e-5."""
+    from pycpet_amd import ops
+
+    x = np.array([[2.0, 1.0, -1.5], [-1.7, 0.4, 2.2], [0.3, -2.5, 0.8]], np.float32)
+    q = np.array([0.3, -0.6, 0.45], np.float32)
+    dims = np.full(3, 1.0, np.float32)
+    starts = np.array([[1.5, 0.0, 0.0], [0.0, -1.2, 0.3], [1.0, 0.0, 0.0], [-1.0, 0.2, 0.1],
+                       [0.3, 1.0, -1.0], [-1.0, -1.0, -1.0], [0.9999, 0.0, 0.0],
+                       [0.0, 0.0, 0.0]], np.float32)
+    n_iter = np.full(len(starts), 400, np.int32)
+    f64, s64, e64 = oracle_mod.topo_f64(starts, n_iter, 0.01, dims, x, q)
+    for prec in ("auto", "f64"):
+        r = ops.topo(starts, n_iter, 0.01, dims, x, q, prec=prec, ctx=gpu_ctx)
+        np.testing.assert_array_equal(r.steps, s64)
+        np.testing.assert_array_equal(r.exited, e64)
+        np.testing.assert_allclose(r.feats, f64, rtol=1e-5, atol=1e-9)
+    assert r.steps[0] == 1 and r.exited[0] == 1
+
+
 def test_full_size_100k(gpu_ctx, oracle_mod):
     """The 5k-charge / 100k-streamline workload at full size: invariants over
     all streamlines plus a seeded 48-streamline sample against the oracle."""

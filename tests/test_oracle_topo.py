@@ -322,3 +322,30 @@ def test_f32_port_radial_lines_and_zero_steps(oracle_mod):
     np.testing.assert_allclose(f32[clean, 0], s32[clean] * float(np.float32(h)), rtol=1e-5)
     assert np.all(f32[:4, 0] == 0.0) and np.all(s32[:4] == 0)
     assert np.abs(f32[:, 1]).max() < 0.5  # FP32 finite differences of straight lines
+
+
+def _boundary_starts():
+    """Starts outside the box, exactly on faces, edges and a corner, and
+    just inside; the reference steps once before its strict box test
+    (math_module.c:534-556)."""
+    return np.array([[1.5, 0.0, 0.0], [0.0, -1.2, 0.3], [1.0, 0.0, 0.0], [-1.0, 0.2, 0.1],
+                     [0.3, 1.0, -1.0], [-1.0, -1.0, -1.0], [0.9999, 0.0, 0.0],
+                     [0.0, 0.0, 0.0]], np.float32)
+
+
+def test_boundary_starts_vs_live_reference(oracle_mod):
+    """Outside / on-face / corner starts: the FP64 oracle's distance and
+    curvature against the reference's compiled thread_operation, and its
+    exit rule (a start outside leaves after one step)."""
+    if not oracle_mod.Ref.available():
+        pytest.skip("reference C not built here")
+    x = np.array([[2.0, 1.0, -1.5], [-1.7, 0.4, 2.2], [0.3, -2.5, 0.8]], np.float32)
+    q = np.array([0.3, -0.6, 0.45], np.float32)
+    dims = np.full(3, 1.0, np.float32)
+    starts = _boundary_starts()
+    n_iter = np.full(len(starts), 400, np.int32)
+    f64, s64, e64 = oracle_mod.topo_f64(starts, n_iter, 0.01, dims, x, q)
+    assert s64[0] == 1 and e64[0] == 1 and s64[1] == 1 and e64[1] == 1
+    want = oracle_mod.Ref().topo(starts, n_iter, 0.01, dims, x, q).astype(float)
+    np.testing.assert_allclose(f64[:, 0], want[:, 0], rtol=2e-4, atol=1e-6)
+    np.testing.assert_allclose(f64[:, 1], want[:, 1], rtol=5e-2, atol=1e-3)
