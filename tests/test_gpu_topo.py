@@ -180,6 +180,29 @@ def test_boundary_starts(gpu_ctx, oracle_mod):
     assert r.steps[0] == 1 and r.exited[0] == 1
 
 
+def test_start_on_a_charge(gpu_ctx, oracle_mod):
+    """A start exactly on a charge: the reference's field there is 0 * inf =
+    NaN (math_module.c:296-333), so its walk never passes the box test and
+    runs all n_iter steps with NaN features; the GPU does the same, the
+    histogram drops the NaN row, and the other streamlines are unaffected."""
+    from pycpet_amd import ops
+
+    x = np.array([[0.2, 0.1, -0.3], [-1.7, 0.4, 2.2]], np.float32)
+    q = np.array([0.3, -0.6], np.float32)
+    dims = np.full(3, 1.0, np.float32)
+    starts = np.array([[0.2, 0.1, -0.3], [0.0, 0.0, 0.0]], np.float32)
+    n_iter = np.array([50, 50], np.int32)
+    f64, s64, e64 = oracle_mod.topo_f64(starts, n_iter, 0.01, dims, x, q)
+    assert np.isnan(f64[0]).all() and s64[0] == 50 and e64[0] == 0
+    edges = (np.linspace(0.0, 1.0, 5), np.linspace(0.0, 1.0, 5))
+    r = ops.topo(starts, n_iter, 0.01, dims, x, q, edges=edges, ctx=gpu_ctx)
+    assert np.isnan(r.feats[0]).all()
+    np.testing.assert_array_equal(r.steps, s64)
+    np.testing.assert_array_equal(r.exited, e64)
+    np.testing.assert_allclose(r.feats[1], f64[1], rtol=1e-5)
+    assert int(r.hist.sum()) == 1
+
+
 def test_full_size_100k(gpu_ctx, oracle_mod):
     """The 5k-charge / 100k-streamline workload at full size: invariants over
     all streamlines plus a seeded 48-streamline sample against the oracle."""
