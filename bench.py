@@ -425,7 +425,8 @@ def forms_rows() -> dict:
     batching (32 mixed MD frames, 16 volume and 16 streamline frames of
     3k-40k charges: one volume and one persistent streamline launch on two
     streams, default shapes, against the per-frame calls at their tuned
-    shapes) and serving (Poisson arrivals at 200 req/s for 10 s of the
+    shapes; and 200 residue-sized streamline frames of 8-24 charges, batched
+    against per-frame calls) and serving (Poisson arrivals at 200 req/s for 10 s of the
     120 s configuration, continuous batching against one request at a
     time).  The full-size runs are `python -m pycpet_amd.framebench batch
     --frames 512` and `serve --duration 120`."""
@@ -437,6 +438,50 @@ def forms_rows() -> dict:
         "frames", "volume_frames", "topo_frames", "pair_evals", "batched_s",
         "batched_frames_per_s", "per_frame_s", "per_frame_frames_per_s",
         "batched_vs_per_frame", "batched_pair_evals_per_s")}
+    # small frames: 200 residue-sized frames (8-24 charges) over the same
+    # 20,000 starts, batched against the per-frame calls at their tuned shapes
+    import torch
+
+    from pycpet_amd import _native as nat, batch as B, ops, workloads as W
+
+    dev = torch.device("cuda:0")
+    dims = np.array([1.5, 1.5, 1.5], np.float32)
+    rst, rit, _ = W.topo_starts(20_000, dims, 0.001, 2000, seed=61, start_seed=62)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    st, it = T(rst), T(rit)
+    rg = np.random.default_rng(31)
+    frames = []
+    for _ in range(200):
+        n = int(rg.integers(8, 25))
+        c = rg.standard_normal(3)
+        c *= rg.uniform(5.0, 14.0) / np.linalg.norm(c)
+        frames.append({"x": T((c + rg.normal(0.0, 1.2, (n, 3))).astype(np.float32)),
+                       "q": T(rg.normal(0.0, 0.4, n).astype(np.float32)), "starts": st,
+                       "n_iter": it, "h": 0.001, "dims": dims})
+    ctx = nat.Context(0)
+    edges = (np.linspace(0.0, 3.2, 21), np.linspace(0.0, 40.0, 21))
+
+    def per_frame():
+        r = [ops.topo(f["starts"], f["n_iter"], f["h"], f["dims"], f["x"], f["q"],
+                      edges=edges, ctx=ctx) for f in frames]
+        torch.cuda.synchronize()
+        return r
+
+    def batched():
+        r = B.batch_topo(frames, edges=edges, ctx=ctx)
+        torch.cuda.synchronize()
+        return r
+
+    per_frame()
+    batched()
+    t_pf, a = _best_of(per_frame, k=2)
+    t_b, b = _best_of(batched, k=2)
+    same = all(torch.equal(x.feats, y.feats) and torch.equal(x.hist, y.hist) for x, y in zip(a, b))
+    out["batching_small_frames"] = {
+        "frames": len(frames), "charges_per_frame": "8-24", "starts_per_frame": 20_000,
+        "per_frame_s": t_pf, "batched_s": t_b, "batched_vs_per_frame": t_pf / t_b,
+        "batched_frames_per_s": len(frames) / t_b, "results_bitwise_equal": bool(same),
+        "note": "device tensors in, 20x20 histograms counted in the launches"}
     sv = FB.run_serve(200.0, 10.0, 4, plain_max_s=20.0)
     keep = ("completed", "rejected", "frames_per_s", "p50_ms", "p99_ms", "wall_s")
     out["serving_200rps_10s"] = {
