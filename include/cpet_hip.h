@@ -363,7 +363,7 @@ int cpet_ctx_refined_points(cpet_ctx *ctx, long long *n);
  *   charge slices in order); 2, 4 and 8 waves split the charges
  *   differently and agree to FP32 rounding.  The default is 1 wave for a
  *   batch whose frames all have at most 128 charges and for a single frame
- *   of at most 16, else 16. */
+ *   of at most 16 charges and at least 50,000 starts, else 16. */
 int cpet_set_tuning(cpet_ctx *ctx, int kind, int a, int b);
 
 /* ------------------------------------------------------ cluster step

@@ -656,7 +656,8 @@ struct FrameView {
 // without changing any result.
 constexpr int kVirtWaves = 16;
 constexpr int kVirtMaxCharges = 128;  // batch default: one-wave walk up to this many
-constexpr int kVirtMaxSingle = 16;    // single-frame default: the same
+constexpr int kVirtMaxSingle = 16;    // single-frame default: the same, with
+constexpr int kVirtMinStarts = 50000;  // at least this many starts
 
 // Runs the workgroup's 64 slots until the frame's queue is exhausted.  Slot
 // state is identical in every wave: each wave sums its charge slice for all
@@ -1393,10 +1394,13 @@ int topo_run(cpet_ctx* ctx, int N, int L, const float* starts,
 
   // far-field records and their safety flag (plain charges, S >= 4)
   // 16 waves x 64 slots: fastest measured (S 4/8/16, G 256..2048); frames
-  // of at most kVirtMaxSingle charges: one-wave workgroups, the same results
-  // (kVirtWaves), 2.2x faster at 16 charges and 100,000 starts, slower at 64
+  // of at most kVirtMaxSingle charges and at least kVirtMinStarts starts:
+  // one-wave workgroups, the same results (kVirtWaves), 2.2x faster at 16
+  // charges and 100,000 starts; slower at 64 charges, and at 20,000 starts,
+  // where the longest streamline's step latency bounds the call (2.2 against
+  // 1.95 ms)
   int S = ctx->tune[2][0];
-  if (S <= 0) S = N <= kVirtMaxSingle ? 1 : 16;
+  if (S <= 0) S = (N <= kVirtMaxSingle && L >= kVirtMinStarts) ? 1 : 16;
   const float4* d_far = nullptr;
   const int* d_far_unsafe = nullptr;
   if (src == kSrcCharge && S >= 4 && ctx->tune[5][0] == 2) {

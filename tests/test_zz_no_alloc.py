@@ -41,7 +41,7 @@ def test_noalloc_equals_cpet_topo(gpu_ctx):
     stream = torch.cuda.current_stream(dev).cuda_stream
     cases = [W.small_topo(N=4000, L=20000, cap=300, seed=11, dims=(1.0, 1.0, 1.0), R=20.0),
              W.small_topo(N=700, L=3000, cap=200, seed=12),
-             W.small_topo(N=12, L=5000, cap=200, seed=15)]  # one-wave workgroups
+             W.small_topo(N=12, L=60_000, cap=200, seed=15)]  # one-wave workgroups
     for c in cases:
         print(f"case noalloc {c.name}", flush=True)
         edges = (np.linspace(0.0, 2.0, 11), np.linspace(0.0, 40.0, 9))
