@@ -264,7 +264,8 @@ def make_histograms_mem(topo_files, output_dir, plot=False,
     721-845): the same shared bins as make_histograms, each frame's
     histogram saved as <output_dir>/<file name without its 4-character
     extension>_hist.npy (the density values, normalised to sum 1), and the
-    list of those paths returned.  The counts run on the GPU."""
+    list of those paths returned.  The counts run on the GPU.  `plot` is
+    accepted and ignored (the reference shows each histogram on screen)."""
     import os
 
     ctx = ctx or nat.default_context()
@@ -290,7 +291,8 @@ def make_histograms_mem(topo_files, output_dir, plot=False,
 
 def make_histograms(topo_files, plot=False, ctx: Optional[nat.Context] = None) -> np.ndarray:
     """Normalised, flattened 2-D histograms of every .top file's features on
-    shared bins; one row per file (CPET/utils/calculator.py:596-718)."""
+    shared bins; one row per file (CPET/utils/calculator.py:596-718).
+    `plot` is accepted and ignored, as the reference ignores it there."""
     feats = _read_all(topo_files)
     return histograms_from_features(feats, ctx=ctx)
 
