@@ -203,6 +203,70 @@ def test_start_on_a_charge(gpu_ctx, oracle_mod):
     assert int(r.hist.sum()) == 1
 
 
+def _random_configurations():
+    """Twelve seeded frames across the shapes the kernels branch on: charge
+    counts from 1 to 2,100 (the four-charge slice split, 64-charge tiles,
+    the padding, one-wave group streaming up to 64 records), anisotropic
+    boxes of half-widths 0.4-1.6 A, h of 0.002-0.02 A, step limits 0-600,
+    charges 0.3-3 A outside the box; 200 starts each."""
+    from pycpet_amd import workloads as W
+
+    rng = np.random.default_rng(2026)
+    frames = []
+    for k, N in enumerate((1, 2, 5, 17, 63, 64, 65, 127, 255, 401, 1000, 2100)):
+        dims = rng.uniform(0.4, 1.6, 3).astype(np.float32)
+        h = float(rng.choice([0.002, 0.005, 0.01, 0.02]))
+        excl = float(rng.uniform(0.3, 3.0))
+        box = float(dims.max())
+        x, q = W.ball_exclusion(N, box + excl + 12.0, box, excl, seed=500 + k)
+        if N == 1:  # the mean-free charge of one would be 0
+            q = np.array([0.7 if k % 2 else -0.7], np.float32)
+        starts = rng.uniform(-dims, dims, (200, 3)).astype(np.float32)
+        n_iter = rng.integers(0, 601, 200).astype(np.int32)
+        frames.append({"x": x, "q": q, "starts": starts, "n_iter": n_iter, "h": h, "dims": dims})
+    return frames
+
+
+def test_random_configurations_vs_oracle(gpu_ctx, oracle_mod):
+    """Each of twelve random frames alone at the default launch shape, and
+    all twelve as one batch at the default shape and with one-wave
+    workgroups forced (every frame, 2,100 charges included), against the
+    FP64 oracle under the matching rule above; the batched results equal
+    the single-frame ones bit for bit; FP64 sums (prec="f64") give the
+    oracle's step counts and stop kinds exactly and its features within
+    1e-8 relative, 1e-10 absolute (a straight line's curvature is FP64
+    rounding noise on both sides, up to about 1e-16 |x| / h^2 = 2.5e-11 at
+    h = 0.002)."""
+    from pycpet_amd import batch, ops
+
+    frames = _random_configurations()
+    single = []
+    for i, f in enumerate(frames):
+        label = f"random frame {i}: N={len(f['q'])} h={f['h']} dims={np.round(f['dims'], 2)}"
+        print(label, flush=True)
+        f64, s64, e64 = oracle_mod.topo_f64(f["starts"], f["n_iter"], f["h"], f["dims"],
+                                            f["x"], f["q"])
+        r = ops.topo(f["starts"], f["n_iter"], f["h"], f["dims"], f["x"], f["q"], ctx=gpu_ctx)
+        compare(r, f64, s64, e64, label)
+        single.append(r)
+        r64 = ops.topo(f["starts"], f["n_iter"], f["h"], f["dims"], f["x"], f["q"], prec="f64",
+                       ctx=gpu_ctx)
+        np.testing.assert_array_equal(r64.steps, s64)
+        np.testing.assert_array_equal(r64.exited, e64)
+        np.testing.assert_allclose(r64.feats, f64, rtol=1e-8, atol=1e-10)
+    try:
+        for S in (0, 1):
+            gpu_ctx.set_tuning(3, S, 0)
+            print(f"random frames batched, waves per workgroup {S or 'default'}", flush=True)
+            outs = batch.batch_topo(frames, ctx=gpu_ctx)
+            for a, b in zip(single, outs):
+                np.testing.assert_array_equal(a.feats, b.feats)
+                np.testing.assert_array_equal(a.steps, b.steps)
+                np.testing.assert_array_equal(a.exited, b.exited)
+    finally:
+        gpu_ctx.set_tuning(3, 0, 0)
+
+
 def test_full_size_100k(gpu_ctx, oracle_mod):
     """The 5k-charge / 100k-streamline workload at full size: invariants over
     all streamlines plus a seeded 48-streamline sample against the oracle."""
