@@ -176,6 +176,55 @@ def test_edge_cases(gpu_ctx, oracle_mod):
         ops.field_grid(np.zeros(0, np.float32), axes[1], axes[2], x, q, ctx=gpu_ctx)
 
 
+def _random_grids():
+    """Ten seeded frames across the shapes the grid kernels branch on:
+    z-lines of 1 to 101 points (points per lane), single points and
+    single lines, 1 to 3,000 charges (64-charge tiles, the charge split
+    over workgroups, the padding), anisotropic boxes of half-widths
+    0.3-2.5 A, charges anywhere around the grid or at least 0.5 or 2 A
+    outside it."""
+    from pycpet_amd import workloads as W
+
+    rng = np.random.default_rng(2027)
+    shapes = [(1, 1, 1), (1, 1, 101), (2, 3, 7), (13, 1, 40), (7, 21, 41), (21, 21, 21),
+              (3, 64, 65), (40, 13, 3), (65, 9, 64), (11, 17, 101)]
+    counts = [1, 3, 64, 65, 257, 1000, 3000, 130, 2048, 511]
+    frames = []
+    for k, (shape, N) in enumerate(zip(shapes, counts)):
+        half = rng.uniform(0.3, 2.5, 3)
+        axes = [np.linspace(-half[i], half[i], shape[i]).astype(np.float32) if shape[i] > 1
+                else np.array([rng.uniform(-0.5, 0.5)], np.float32) for i in range(3)]
+        excl = float(rng.choice([0.0, 0.5, 2.0]))
+        x, q = W.ball_exclusion(N, float(half.max()) + excl + 10.0, float(half.max()), excl,
+                                seed=900 + k)
+        if N == 1:  # the mean-free charge of one would be 0
+            q = np.array([0.9], np.float32)
+        frames.append({"axes": axes, "x": x, "q": q})
+    return frames
+
+
+def test_random_grids_vs_oracle(gpu_ctx, oracle_mod):
+    """E and phi of ten random frames in the default (auto) precision, each
+    alone and all ten as one batch, against the FP64 oracle within the
+    pointwise bound above."""
+    from pycpet_amd import batch, ops
+
+    frames = _random_grids()
+    refs = []
+    for i, f in enumerate(frames):
+        label = f"random grid {i}: shape {tuple(len(a) for a in f['axes'])} N={len(f['q'])}"
+        print(label, flush=True)
+        E64, p64 = oracle_mod.field_f64(mesh(f["axes"]), f["x"], f["q"], want_phi=True)
+        refs.append((E64, p64))
+        E, phi = ops.field_grid(*f["axes"], f["x"], f["q"], want_phi=True, ctx=gpu_ctx)
+        eE, eP = rel_err(E, E64)[0], rel_err(phi, p64)[0]
+        print(f"{label}: E {eE:.2e} phi {eP:.2e}", flush=True)
+        assert eE <= TOL and eP <= TOL
+    print("random grids batched", flush=True)
+    for (E, phi), (E64, p64) in zip(batch.batch_volume(frames, want_phi=True, ctx=gpu_ctx), refs):
+        assert rel_err(E, E64)[0] <= TOL and rel_err(phi, p64)[0] <= TOL
+
+
 def test_floor_near_grid_points(gpu_ctx, oracle_mod):
     """The volume field's r^2 >= 1e-6 floor (math_module.c:405-451) with E
     alone: the grid kernel skips the floor only when no charge is within
