@@ -39,6 +39,41 @@ def test_dipole_vs_oracle(gpu_ctx, oracle_mod, physical):
     assert rel[match, 0].max() <= 1e-4 and rel[match, 1].max() <= 1e-4
 
 
+def test_random_dipole_frames_vs_oracle(gpu_ctx, oracle_mod):
+    """The twelve random streamline frames of the charge tests (1 to 2,100
+    sources, anisotropic boxes, h 0.002-0.02 A) with random moments, in both
+    field forms, against the FP64 oracle under the bar above; sixteen-wave
+    and one-wave workgroups give the same results bit for bit."""
+    from pycpet_amd import ops
+    from test_gpu_topo import _random_configurations
+
+    try:
+        for i, f in enumerate(_random_configurations()):
+            mu = np.random.default_rng(3000 + i).standard_normal((len(f["q"]), 3))
+            mu = (0.5 * mu).astype(np.float32)
+            for physical in (False, True):
+                label = f"random dipole frame {i}: N={len(mu)} h={f['h']} physical={physical}"
+                print(label, flush=True)
+                f64, s64, e64 = oracle_mod.topo_dipole_f64(
+                    f["starts"], f["n_iter"], f["h"], f["dims"], f["x"], mu, physical=physical)
+                runs = []
+                for S in (16, 1):
+                    gpu_ctx.set_tuning(2, S, 0)
+                    runs.append(ops.topo_dipole(f["starts"], f["n_iter"], f["h"], f["dims"],
+                                                f["x"], mu, physical=physical, ctx=gpu_ctx))
+                r = runs[0]
+                for a in ("feats", "steps", "exited"):
+                    np.testing.assert_array_equal(getattr(runs[1], a), getattr(r, a))
+                match = (r.steps == s64) & (r.exited == e64)
+                rel = np.abs(r.feats - f64) / np.maximum(np.abs(f64), 1e-6)
+                print(f"{label}: match {match.mean():.5f} dist max {rel[match, 0].max():.3e} "
+                      f"curv max {rel[match, 1].max():.3e}", flush=True)
+                assert match.mean() >= 0.99
+                assert rel[match, 0].max() <= 1e-4 and rel[match, 1].max() <= 1e-4
+    finally:
+        gpu_ctx.set_tuning(2, 0, 0)
+
+
 def test_dipole_on_axis_exact(gpu_ctx, oracle_mod):
     from pycpet_amd import ops
 
