@@ -155,7 +155,10 @@ def topo_dipole_f64(starts, n_iter, h, dims, x, mu, *, physical=False):
 
 
 def topo_f32(starts, n_iter, h, dims, x, q):
-    """FP32 port of thread_operation (restated CPU baseline)."""
+    """FP32 port of thread_operation (restated CPU baseline).  Pinned by
+    tests/test_oracle_topo.py (the reference C's FP32 outputs in
+    tests/golden/ref_topo_200.npz, the live source-built reference where
+    present, radial lines and zero steps)."""
     starts, x, q, dims = _f32(starts, 3), _f32(x, 3), _f32(q), _f32(dims)
     n_iter = np.ascontiguousarray(n_iter, dtype=np.int32).ravel()
     L, N = starts.shape[0], q.size
@@ -320,7 +323,10 @@ def histogram2d(a, b, edges_a, edges_b):
 
 def box_basis(center, xpt, ypt):
     """Box frame T = [x^; y^; z^] with z^ = x^ x y^, y^ = z^ x x^
-    (CPET/utils/calculator.py:200-215)."""
+    (CPET/utils/calculator.py:200-215).  Pinned by
+    tests/test_oracle_host.py::test_box_basis_closed_forms (identity and
+    cyclic-permutation frames, the y point's x^ component removed,
+    right-handedness)."""
     x = np.asarray(xpt) - np.asarray(center)
     y = np.asarray(ypt) - np.asarray(center)
     xu = x / np.linalg.norm(x)
@@ -422,7 +428,10 @@ def make_histograms_files(topo_files):
     parse of every row into the global arrays (:618-649), then per file a
     second parse for its histogram (:690-707); bins from scipy iqr and the
     global min / max.  Returns (F, B).  Timed as the CPU counterpart of the
-    cluster step from files; its arithmetic is cluster_histograms'."""
+    cluster step from files; its arithmetic is cluster_histograms'.  Pinned
+    by tests/test_oracle_cluster.py (test_oracle_files_path_equals_in_memory:
+    equal to cluster_histograms on the same rows, whose bins and counts are
+    pinned to numpy and scipy there)."""
     from scipy.stats import iqr
 
     lens = np.zeros(len(topo_files), dtype=int)
@@ -506,7 +515,9 @@ def cluster_histograms_mem(features):
 def chi2_matrix_mem(hists):
     """construct_distance_matrix_mem (CPET/utils/calculator.py:981-1000) on
     in-memory histograms: distance_numpy over every pair i < j, mirrored,
-    zero diagonal."""
+    zero diagonal.  Pinned by tests/test_oracle_cluster.py
+    (test_oracle_chi2_mem_equals_pairwise_path: equal to chi2_matrix, whose
+    closed forms pin the pair term)."""
     n = len(hists)
     m = np.zeros((n, n), dtype=np.float64)
     for i in range(n):
@@ -517,7 +528,10 @@ def chi2_matrix_mem(hists):
 
 
 def chi2_distance(h1, h2):
-    """distance_numpy, CPET/utils/calculator.py:975-978."""
+    """distance_numpy, CPET/utils/calculator.py:975-978.  Pinned through
+    chi2_matrix, which applies it to every pair: its closed forms in
+    tests/test_oracle_cluster.py (identical -> 0, disjoint supports -> 1, a
+    hand-computed pair, symmetry)."""
     a = (h1 - h2) ** 2
     b = h1 + h2
     return np.sum(np.divide(a, b, out=np.zeros_like(a), where=b != 0)) / 2.0

@@ -88,3 +88,29 @@ def test_filter_atomspec_kat(oracle_mod):
     # union form: any listed attribute value matches
     drop = oracle_mod.filter_atomspec_drop(ID, {"resid": ["WAT"], "atom_type": ["NI"]}, False)
     assert drop == [0, 1, 3]
+
+
+def test_box_basis_closed_forms(oracle_mod):
+    """box_basis (CPET/utils/calculator.py:200-215) on frames with known
+    answers: rows are x^, y^, z^ with z^ = x^ x y^ and y^ = z^ x x^, so the
+    y point's component along x^ is removed and the frame is right-handed."""
+    c = np.array([1.0, 2.0, 3.0])
+    # axis-aligned points: the identity, for any centre and distances
+    np.testing.assert_allclose(oracle_mod.box_basis(c, c + [2.0, 0, 0], c + [0, 3.0, 0]),
+                               np.eye(3), atol=1e-15)
+    # a y point leaning along x: only its part orthogonal to x^ counts
+    np.testing.assert_allclose(oracle_mod.box_basis(c, c + [1.0, 0, 0], c + [5.0, 1.0, 0]),
+                               np.eye(3), atol=1e-15)
+    # x along +y, y along +z: z^ = e_y x e_z = e_x, rows (e_y, e_z, e_x)
+    want = np.array([[0.0, 1.0, 0.0], [0.0, 0.0, 1.0], [1.0, 0.0, 0.0]])
+    np.testing.assert_allclose(oracle_mod.box_basis(c, c + [0, 0.7, 0], c + [0, 0.2, 1.3]),
+                               want, atol=1e-15)
+    # random frames: orthonormal rows, determinant +1, x^ along the x point
+    rng = np.random.default_rng(11)
+    for _ in range(20):
+        xp, yp = c + rng.standard_normal(3), c + rng.standard_normal(3)
+        T = oracle_mod.box_basis(c, xp, yp)
+        np.testing.assert_allclose(T @ T.T, np.eye(3), atol=1e-12)
+        assert abs(np.linalg.det(T) - 1.0) < 1e-12
+        np.testing.assert_allclose(T[0], (xp - c) / np.linalg.norm(xp - c), atol=1e-15)
+        assert T[1] @ (yp - c) > 0 and abs(T[2] @ (yp - c)) < 1e-12
