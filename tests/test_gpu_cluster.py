@@ -36,6 +36,30 @@ def test_histograms_bit_exact(gpu_ctx, oracle_mod, ragged):
     np.testing.assert_array_equal(got, want)
 
 
+def test_histograms_with_empty_frames(gpu_ctx, oracle_mod):
+    """Frames with no rows (empty .top files) first, in the middle and last:
+    their histograms are 0 / 0 = NaN in every bin, as numpy's normalisation
+    gives, and every other frame, the shared bins and the reference's
+    mean-length rule for unequal frames are as the oracle has them."""
+    import warnings
+
+    from pycpet_amd import clusterhist as CH
+
+    feats = _frames(5, 6, 1500, ragged=True)
+    feats.insert(0, np.zeros((0, 2)))
+    feats.insert(3, np.zeros((0, 2)))
+    feats.append(np.zeros((0, 2)))
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")  # the unequal-sizes warning; 0 / 0
+        want, _ = oracle_mod.cluster_histograms(feats)
+        got = CH.histograms_from_features(feats, ctx=gpu_ctx)
+    empty = [k for k, f in enumerate(feats) if len(f) == 0]
+    print(f"F={len(feats)} empty frames {empty}", flush=True)
+    assert empty == [0, 3, len(feats) - 1]
+    assert np.isnan(got[empty]).all() and not np.isnan(np.delete(got, empty, 0)).any()
+    np.testing.assert_array_equal(got, want)
+
+
 def test_chi2_matrix_vs_sklearn(gpu_ctx, oracle_mod):
     from pycpet_amd import clusterhist as CH
 
