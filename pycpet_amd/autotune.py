@@ -22,7 +22,6 @@ from __future__ import annotations
 import json
 import math
 import os
-import time
 from pathlib import Path
 from typing import Callable, Dict, Iterable, Optional, Tuple
 

@@ -10,7 +10,6 @@ call is queued on torch's current stream).
 """
 from __future__ import annotations
 
-import ctypes
 from typing import List, Optional, Sequence
 
 import numpy as np

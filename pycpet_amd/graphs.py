@@ -22,8 +22,6 @@ counts and grid / streamline counts are fixed by the capture.
 """
 from __future__ import annotations
 
-from typing import Any, Callable, Dict, Optional, Tuple
-
 from . import _native as nat
 
 

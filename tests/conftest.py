@@ -1,6 +1,5 @@
 """Shared pytest setup: the `gpu` marker, repo root on sys.path, and helpers
 to tell whether the oracle / reference build / GPU library are usable."""
-import os
 import sys
 from pathlib import Path
 

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from conftest import GOLDEN
-from pdbgen import synthetic_frame, write_pdb
+from pdbgen import synthetic_frame
 
 
 def test_default_options_and_validation():
