@@ -302,6 +302,22 @@ def test_rk4_vs_oracle(gpu_ctx, oracle_mod, N, L):
     compare(r, f64, s64, e64, f"rk4_{N}x{L}")
 
 
+def test_rk4_random_configurations_vs_oracle(gpu_ctx, oracle_mod):
+    """integrator="rk4" on the twelve random frames (1 to 2,100 charges,
+    anisotropic boxes, h 0.002-0.02 A) against the FP64 RK4 oracle, with the
+    same matching rule."""
+    from pycpet_amd import ops
+
+    for i, f in enumerate(_random_configurations()):
+        label = f"rk4 random frame {i}: N={len(f['q'])} h={f['h']}"
+        print(label, flush=True)
+        r = ops.topo(f["starts"], f["n_iter"], f["h"], f["dims"], f["x"], f["q"],
+                     integrator="rk4", ctx=gpu_ctx)
+        f64, s64, e64 = oracle_mod.topo_rk4_f64(f["starts"], f["n_iter"], f["h"], f["dims"],
+                                                f["x"], f["q"])
+        compare(r, f64, s64, e64, label)
+
+
 def test_rk4_radial_lines_equal_euler(gpu_ctx):
     """A single charge: straight radial lines, so RK4 and Euler take the same
     steps (the RK4 stage bookkeeping must not change the step count)."""
